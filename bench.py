@@ -1,0 +1,280 @@
+#!/usr/bin/env python3
+"""Benchmark of the Pauli-operator path on one MI355X.
+
+Default workload (the headline): 20 qubits, 2000 random Pauli terms
+(labels uniform over {I,X,Y,Z}, complex N(0,1) coefficients): one step is the
+full CSR build (2^20 rows x G distinct x masks, uint64 indices, complex128
+values) followed by one complex128 CSR matvec on a normalised random state.
+`value` is whole-step throughput in CSR nonzeros per second; inputs are
+resident in device memory when timing starts.
+
+Also reported (same JSON line, key "extra"): the 24-qubit Heisenberg ring
+matrix-free apply on 8 states, FP64 vs mixed precision (GB/s, % of HBM peak,
+speedup, error vs FP64), unless --no-extra.
+
+roofline: the dominant kernel's algorithmic bytes per launch / its mean
+duration from HIP events on the launch stream.  cpu_baseline: the oracle's C
+restatement of qrusty's RowwiseUnsafeChunked/1000 build + spmat_dot_densevec
+on a bounded sample of rows, on up to 16 host threads.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import statistics
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def load_traffic(kernel: str, workload: str):
+    path = os.path.join(HERE, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return t.get(f"{kernel}@{workload}")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_threads() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+# ---------------------------------------------------------------------------
+def bench_csr(args, torch, qa):
+    from qrusty_amd import _lib as L
+    from qrusty_amd import workloads
+
+    nq, T = 20, 2000
+    labels, coeffs = workloads.random_op(nq, T, seed=workloads.SEED_CFG2)
+    op = qa.SparsePauliOp([qa.Pauli(l) for l in labels], coeffs)
+    h = op._handle(L.QR_PHASE_ROWWISE)
+    N, G, nnz = 1 << nq, h.G, h.nnz
+    log(f"csr workload: n={nq} T={T} G={G} nnz={nnz} ({24 * nnz / 1e9:.1f} GB CSR)")
+    dev = torch.device("cuda", 0)
+    indptr = torch.empty(N + 1, dtype=torch.int64, device=dev)
+    indices = torch.empty(nnz, dtype=torch.int64, device=dev)
+    data = torch.empty(nnz, dtype=torch.complex128, device=dev)
+    x = torch.from_numpy(workloads.random_states(N, 1, seed=workloads.SEED_CFG2 + 1)[0]).to(dev)
+    y = torch.empty(N, dtype=torch.complex128, device=dev)
+    stream = torch.cuda.current_stream()
+    sh = ctypes.c_void_p(stream.cuda_stream)
+    P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+
+    def step(ev=None):
+        if ev:
+            ev[0].record(stream)
+        L.check(L.lib.qr_csr_build(h.h, P(indptr), P(indices), P(data), sh))
+        if ev:
+            ev[1].record(stream)
+        L.check(L.lib.qr_csr_matvec(N, N, nnz, P(indptr), P(indices), P(data), P(x), P(y), sh))
+        if ev:
+            ev[2].record(stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(evs[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    build_ms = [e[0].elapsed_time(e[1]) for e in evs]
+    mv_ms = [e[1].elapsed_time(e[2]) for e in evs]
+    ms_per_step = 1e3 * wall / args.steps
+    b_build = 24 * nnz + 8 * (N + 1)
+    b_mv = 24 * nnz + 8 * (N + 1) + 32 * N
+    bm, mm = statistics.mean(build_ms), statistics.mean(mv_ms)
+    res = {
+        "workload": "csr_build+matvec n=20 T=2000 random", "N": N, "T": T, "G": G, "nnz": nnz,
+        "ms_per_step": ms_per_step, "value": nnz / (ms_per_step / 1e3),
+        "build_ms": bm, "matvec_ms": mm,
+        "build_nnz_per_s": nnz / (bm / 1e3), "build_GBps": b_build / bm / 1e6,
+        "matvec_GBps": b_mv / mm / 1e6,
+    }
+    res["build_pct_hbm"] = 100 * res["build_GBps"] / HBM_PEAK_GBPS
+    res["matvec_pct_hbm"] = 100 * res["matvec_GBps"] / HBM_PEAK_GBPS
+    dom = ("k_csr_build", bm, b_build) if bm >= mm else ("k_csr_matvec", mm, b_mv)
+    res["roofline"] = {
+        "kernel": dom[0], "bound": "hbm", "achieved": dom[2] / dom[1] / 1e6, "peak": HBM_PEAK_GBPS,
+        "unit": "GB/s", "frac": dom[2] / dom[1] / 1e6 / HBM_PEAK_GBPS,
+        "traffic": load_traffic(dom[0], "n20_T2000"),
+        "algorithmic_bytes": dom[2], "mean_ms": dom[1],
+    }
+    log(f"build {bm:.3f} ms ({res['build_GBps']:.0f} GB/s), matvec {mm:.3f} ms "
+        f"({res['matvec_GBps']:.0f} GB/s), step {ms_per_step:.3f} ms")
+
+    # parity spot check of the timed output (sampled rows vs the oracle)
+    if not args.no_check:
+        from oracle import oracle as orc
+        import numpy as np
+        p = orc.Params(labels, coeffs)
+        rows = np.random.default_rng(1).choice(N, size=64, replace=False)
+        ind_h = indices.view(N, G)[torch.from_numpy(rows).to(dev)].cpu().numpy().view(np.uint64)
+        dat_h = data.view(N, G)[torch.from_numpy(rows).to(dev)].cpu().numpy()
+        ok = True
+        for i, r in enumerate(rows):
+            c, v = orc.make_row(p, int(r))
+            ok &= (c.tobytes() == ind_h[i].tobytes()) and (v.tobytes() == dat_h[i].tobytes())
+        res["parity_sampled_rows_bit_exact"] = bool(ok)
+    del indices, data
+    torch.cuda.empty_cache()
+    return res, (labels, coeffs)
+
+
+def cpu_baseline_csr(labels, coeffs, target_s=4.0):
+    """Oracle (C restatement of RowwiseUnsafeChunked/1000 + spmat_dot_densevec)
+    on a bounded sample of rows; median of 3 runs after 1 warm-up."""
+    import numpy as np
+    from oracle import oracle as orc
+    from qrusty_amd import workloads
+    p = orc.Params(labels, coeffs)
+    nthr = cpu_threads()
+    x = workloads.random_states(p.N, 1, seed=workloads.SEED_CFG2 + 1)[0]
+
+    def run(rows):
+        t0 = time.perf_counter()
+        ip, ind, dat = orc.build_csr(p, 0, rows, step=1000, nthreads=nthr)
+        orc.spmat_dot_densevec(ip, ind, dat, x, nthreads=nthr)
+        return time.perf_counter() - t0, len(ind)
+
+    rows = 4096
+    t, _ = run(rows)
+    rows = int(min(p.N, max(4096, rows * target_s / max(t, 1e-3))))
+    rows = (rows // 1000) * 1000 or 1000
+    run(rows)  # warm-up
+    times = []
+    for _ in range(3):
+        t, nnz = run(rows)
+        times.append(t)
+    med = statistics.median(times)
+    return {"value": nnz / med, "unit": "nnz/s", "cores": nthr, "kind": "port",
+            "sample": f"rows 0..{rows} of 2^20 ({nnz} nnz): RowwiseUnsafeChunked/1000 build "
+                      f"+ spmat_dot_densevec, median of 3 after 1 warm-up, {nthr} threads"}
+
+
+# ---------------------------------------------------------------------------
+def bench_apply(args, torch, qa):
+    """24-qubit Heisenberg ring, 8 states: FP64 vs mixed matrix-free apply."""
+    import numpy as np
+    from qrusty_amd import _lib as L
+    from qrusty_amd import workloads
+
+    nq, nvec = 24, 8
+    labels, coeffs = workloads.heisenberg_ring(nq, seed=workloads.SEED_CFG3)
+    op = qa.SparsePauliOp([qa.Pauli(l) for l in labels], coeffs)
+    h = op._handle(L.QR_PHASE_ROWWISE)
+    N = 1 << nq
+    dev = torch.device("cuda", 0)
+    x64 = torch.from_numpy(workloads.random_states(N, nvec, seed=workloads.SEED_CFG3 + 1)).to(dev)
+    x32 = x64.to(torch.complex64)
+    y64 = torch.empty_like(x64)
+    y32 = torch.empty_like(x32)
+    e64 = torch.empty(2 * nvec, dtype=torch.float64, device=dev)
+    e32 = torch.empty(2 * nvec, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream()
+    sh = ctypes.c_void_p(stream.cuda_stream)
+    P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+
+    def timed(prec, xt, yt, et, reps):
+        for _ in range(2):
+            L.check(L.lib.qr_pauli_apply(h.h, prec, nvec, P(xt), P(yt), P(et), sh))
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        a.record(stream)
+        for _ in range(reps):
+            L.check(L.lib.qr_pauli_apply(h.h, prec, nvec, P(xt), P(yt), P(et), sh))
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+    reps = max(3, args.steps)
+    ms64 = timed(L.QR_PREC_FP64, x64, y64, e64, reps)
+    ms32 = timed(L.QR_PREC_MIXED, x32, y32, e32, reps)
+    b64 = nvec * N * 32
+    b32 = nvec * N * 16
+    ry = (torch.linalg.vector_norm((y32.to(torch.complex128) - y64).reshape(-1)) /
+          torch.linalg.vector_norm(y64.reshape(-1))).item()
+    E64 = e64.cpu().numpy().view(np.complex128)
+    E32 = e32.cpu().numpy().view(np.complex128)
+    rE = float(np.max(np.abs(E32 - E64) / np.abs(E64)))
+    res = {
+        "workload": "heisenberg ring n=24, 8 states, matrix-free apply + <psi|H|psi>",
+        "G": h.G, "T": h.T,
+        "fp64_ms": ms64, "fp64_GBps": b64 / ms64 / 1e6, "fp64_pct_hbm": 100 * b64 / ms64 / 1e6 / HBM_PEAK_GBPS,
+        "mixed_ms": ms32, "mixed_GBps": b32 / ms32 / 1e6, "mixed_pct_hbm": 100 * b32 / ms32 / 1e6 / HBM_PEAK_GBPS,
+        "mixed_speedup_vs_fp64": ms64 / ms32,
+        "mixed_rel_err_y_vs_fp64": ry, "mixed_rel_err_expect_vs_fp64": rE,
+        "algorithmic_bytes": {"fp64": b64, "mixed": b32},
+    }
+    log(f"apply fp64 {ms64:.3f} ms ({res['fp64_GBps']:.0f} GB/s) mixed {ms32:.3f} ms "
+        f"({res['mixed_GBps']:.0f} GB/s) speedup {ms64 / ms32:.2f}x err y {ry:.2e} E {rE:.2e}")
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-extra", action="store_true", help="skip the Heisenberg apply workload")
+    ap.add_argument("--no-check", action="store_true", help="skip the sampled parity check")
+    ap.add_argument("--only", choices=["csr", "apply"], default=None)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
+              "(see DESIGN.md for how the path would shard).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import qrusty_amd as qa
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    torch.cuda.set_device(0)
+
+    out = {"metric": "CSR nnz/s + Pauli-sum matvec GB/s (% HBM peak); FP64 vs mixed @<=1e-6 rel",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "c128/u64", "data": "synthetic"}
+    extra = {}
+    if args.only in (None, "csr"):
+        res, (labels, coeffs) = bench_csr(args, torch, qa)
+        out.update({"value": res["value"], "unit": "nnz/s", "ms_per_step": res["ms_per_step"],
+                    "config": {"workload": res["workload"], "num_qubits": 20, "num_terms": 2000,
+                               "groups": res["G"], "nnz": res["nnz"], "seed": "0x51520002"},
+                    "roofline": res.pop("roofline")})
+        extra["csr"] = res
+        if not args.no_cpu:
+            log("cpu baseline ...")
+            out["cpu_baseline"] = cpu_baseline_csr(labels, coeffs)
+            log(f"cpu baseline {out['cpu_baseline']['value']:.3e} nnz/s")
+    if args.only in (None, "apply") and not args.no_extra:
+        extra["apply"] = bench_apply(args, torch, qa)
+        if args.only == "apply":
+            a = extra["apply"]
+            out.update({"value": a["mixed_GBps"], "unit": "GB/s", "ms_per_step": a["mixed_ms"],
+                        "dtype": "c64/fp32+TwoSum", "config": {"workload": a["workload"]}})
+    out["extra"] = extra
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

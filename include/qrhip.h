@@ -1,0 +1,171 @@
+/*
+ * qrhip.h -- C ABI of the MI355X (gfx950) Pauli-operator library libqrhip.so.
+ *
+ * The library replaces qrusty's rowwise CSR build and CSR matvec
+ * (qrusty/src/accel.rs:141-370, reached from qrusty/src/lib.rs:552-585 and
+ * pyqrusty/src/lib.rs:406-415, 476-491) with hand-written HIP kernels, and
+ * adds a matrix-free Pauli-sum apply, <psi|H|psi>, and a batched launch.
+ *
+ * Conventions
+ *  - Every function returns 0 (QR_OK) or a negative QR_ERR_* code; the message
+ *    of the last failure on the calling thread is returned by qr_last_error().
+ *    No C++ exception crosses this boundary.
+ *  - Device pointers are plain device addresses on the current HIP device.
+ *    Complex values are interleaved (re, im): complex128 = 2 doubles,
+ *    complex64 = 2 floats.  Indices and indptr are uint64, as in the
+ *    reference's CsMatI<Complex64, u64, u64> (qrusty/src/lib.rs:562-570).
+ *  - `stream` is a hipStream_t (NULL = the default stream).  Calls are
+ *    stream-ordered and asynchronous unless stated otherwise.
+ *  - A qr_op may be used from several threads for launches; creating and
+ *    destroying it is not concurrent with its use.
+ */
+#ifndef QRHIP_H
+#define QRHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QRHIP_ABI_VERSION 1
+
+enum qr_status {
+    QR_OK = 0,
+    QR_ERR_ARG = -1,      /* invalid argument (null pointer, bad size, ...)      */
+    QR_ERR_LABEL = -2,    /* malformed Pauli label (lib.rs:129)                 */
+    QR_ERR_TERMS = -3,    /* SparsePauliOp::new validation (lib.rs:354-376)     */
+    QR_ERR_MODE = -4,     /* unrecognized AccelMode string (lib.rs:327-329)     */
+    QR_ERR_SIZE = -5,     /* problem too large for the requested operation      */
+    QR_ERR_HIP = -6,      /* a HIP runtime call failed                          */
+    QR_ERR_NOMEM = -7     /* allocation failed                                  */
+};
+
+/* Message of the last failure on this thread ("" if none). */
+const char *qr_last_error(void);
+
+/* QRHIP_ABI_VERSION of the loaded library. */
+int qr_abi_version(void);
+
+/* ------------------------------------------------------------------------
+ * Pauli labels.  Replaces Pauli::new / parse_label / x_indices / z_indices /
+ * phase (qrusty/src/lib.rs:125-181): label ^([+-]?)1?([ij]?)([IXYZ]+)$, qubit
+ * q is label[len-1-q]; x bit = X or Y, z bit = Y or Z;
+ * base_phase = 2*(sign=='-') + (i|j present); phase = (base_phase + #Y) mod 4.
+ * Labels of more than 63 qubits are rejected with QR_ERR_SIZE (the reference
+ * overflows its u64 masks there).  Host only; no GPU needed.
+ * ---------------------------------------------------------------------- */
+int qr_pauli_parse(const char *label, uint32_t *num_qubits, uint32_t *base_phase,
+                   uint32_t *phase, uint64_t *x_mask, uint64_t *z_mask);
+
+/* ------------------------------------------------------------------------
+ * Operators.  Replaces SparsePauliOp::new / from_labels (lib.rs:354-393) plus
+ * accel::rowwise::make_params (accel.rs:141-157): each term becomes
+ * (x, z, c' = M[p] * c) with M = {1, -i, -1, +i}.
+ *
+ * phase_convention selects p:
+ *   QR_PHASE_ROWWISE  p = (base_phase + #Y) mod 4   -- the rowwise/accel
+ *                     builders (accel.rs:145-153) and Pauli::to_matrix_accel
+ *   QR_PHASE_KRON     p = (#Y - base_phase) mod 4   -- Pauli::to_matrix, i.e.
+ *                     i^base_phase times the Kronecker product (lib.rs:182-213)
+ * The two agree for labels without an i/j prefix.
+ *
+ * Errors (messages as the reference's QrustyErr strings):
+ *   QR_ERR_TERMS "SparsePauliOp::new: at least one pauli must be supplied"
+ *   QR_ERR_TERMS "SparsePauliOp::new: all supplied paulis must have the same #qubits"
+ *   QR_ERR_LABEL "error: malformed label"
+ * (a length mismatch of labels/coeffs cannot occur: both have num_terms).
+ * Host only; device tables are uploaded on first GPU use.
+ * ---------------------------------------------------------------------- */
+typedef struct qr_op qr_op;
+
+enum qr_phase_convention { QR_PHASE_ROWWISE = 0, QR_PHASE_KRON = 1 };
+
+int qr_op_create(const char *const *labels, const double *coeffs /* 2*num_terms */,
+                 uint64_t num_terms, int32_t phase_convention, qr_op **out);
+int qr_op_destroy(qr_op *op);
+
+/* num_qubits n, terms T, distinct x masks G, CSR nnz = 2^n * G (every row of
+ * the rowwise CSR holds exactly G entries, explicit zeros kept; accel.rs:191-208).
+ * nnz is 0 when 2^n * G overflows 64 bits. */
+int qr_op_info(const qr_op *op, uint32_t *num_qubits, uint64_t *num_terms,
+               uint64_t *num_groups, uint64_t *nnz);
+
+/* The make_params table in input order: x[T], z[T], coeff[2T] (c').  Host only. */
+int qr_op_terms(const qr_op *op, uint64_t *x, uint64_t *z, double *coeff);
+
+/* Column order of one CSR row as the device kernels lay it out: cols[G]
+ * (ascending r ^ x_g) and the group index of each position.  Host only; an
+ * inspection aid for the row-ordering tables (no values are computed). */
+int qr_op_row_layout(const qr_op *op, uint64_t row, uint64_t *cols, uint32_t *groups);
+
+/* ------------------------------------------------------------------------
+ * AccelMode strings.  Replaces AccelMode::try_from (lib.rs:293-331):
+ * "" | Binary | Accel | Rowwise | RowwiseUnsafe | RowwiseUnsafeChunked/N |
+ * Reduce | Rayon | RayonChunked/N, plus "Hip" for this library.  Every mode
+ * is served by the HIP rowwise kernel; *phase_convention reports which phase
+ * convention that mode uses in the reference (lib.rs:401-585).
+ * Unknown strings -> QR_ERR_MODE "to_matrix_mode: unrecognized mode <s>"
+ * (pyqrusty/src/lib.rs:410-411).  Host only.
+ * ---------------------------------------------------------------------- */
+enum qr_mode {
+    QR_MODE_DEFAULT = 0, QR_MODE_BINARY = 1, QR_MODE_ACCEL = 2, QR_MODE_ROWWISE = 3,
+    QR_MODE_ROWWISE_UNSAFE = 4, QR_MODE_ROWWISE_UNSAFE_CHUNKED = 5, QR_MODE_REDUCE = 6,
+    QR_MODE_RAYON = 7, QR_MODE_RAYON_CHUNKED = 8, QR_MODE_HIP = 9
+};
+int qr_mode_parse(const char *mode, int32_t *kind, uint64_t *step, int32_t *phase_convention);
+
+/* ------------------------------------------------------------------------
+ * CSR build.  Replaces accel::rowwise::make_unsafe_vectors_chunked
+ * (accel.rs:267-336) and its per-row make_row (accel.rs:171-210).
+ * Row r holds, at positions [r*G, (r+1)*G), the columns r ^ x_g in ascending
+ * order; the value of group g is the term-order fold (first value, then +)
+ * of (-1)^popcount(r & z_t) * c'_t over the terms t with x_t = x_g.
+ * Output is bit-identical to the reference's rowwise CSR.
+ *   d_indptr  : N+1 uint64 (indptr[r] = r*G)
+ *   d_indices : N*G uint64
+ *   d_data    : N*G complex128
+ * qr_csr_build_rows writes rows [row_begin, row_end) only, at positions
+ * (r - row_begin)*G (no indptr); it lets callers stream matrices larger
+ * than device memory.
+ * ---------------------------------------------------------------------- */
+int qr_csr_build(const qr_op *op, uint64_t *d_indptr, uint64_t *d_indices, double *d_data,
+                 void *stream);
+int qr_csr_build_rows(const qr_op *op, uint64_t row_begin, uint64_t row_end,
+                      uint64_t *d_indices, double *d_data, void *stream);
+
+/* ------------------------------------------------------------------------
+ * CSR x dense vector.  Replaces accel::rowwise::spmat_dot_densevec
+ * (accel.rs:338-370): y[r] = sum_k data[k] * x[indices[k]], k in row r, for
+ * any CSR with uint64 indptr/indices (e.g. one made by SpMat.new_unchecked).
+ * complex128 in and out; FP64 accumulation with fused multiply-adds.
+ * nnz (= indptr[nrows]) sizes the team of lanes that shares a row.
+ * ---------------------------------------------------------------------- */
+int qr_csr_matvec(uint64_t nrows, uint64_t ncols, uint64_t nnz, const uint64_t *d_indptr,
+                  const uint64_t *d_indices, const double *d_data, const double *d_x,
+                  double *d_y, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Matrix-free Pauli-sum apply (no counterpart in the reference; equal to the
+ * CSR build followed by the CSR matvec):
+ *   y_v[r] = sum_g D_g(r) * x_v[r ^ x_g],  D_g(r) = sum_{t in g} (-1)^popcount(r&z_t) c'_t
+ * for nvec state vectors stored one after another (x_v at offset v*2^n).
+ *   QR_PREC_FP64  : x, y complex128; FP64 throughout.
+ *   QR_PREC_MIXED : x, y complex64; D_g(r) merged in FP64 and rounded to FP32,
+ *                   products and the sum over groups in FP32 with TwoSum
+ *                   compensation; <x|y> reduced in FP64.
+ * d_expect (optional, may be NULL): 2*nvec doubles (device) receiving
+ * <x_v | y_v> = sum_r conj(x_v[r]) y_v[r] in FP64.
+ * d_y may be NULL when d_expect is given: then only <x|H|x> is produced and
+ * y is never written.
+ * ---------------------------------------------------------------------- */
+enum qr_precision { QR_PREC_FP64 = 0, QR_PREC_MIXED = 1 };
+
+int qr_pauli_apply(const qr_op *op, int32_t precision, uint32_t nvec, const void *d_x,
+                   void *d_y, double *d_expect, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QRHIP_H */

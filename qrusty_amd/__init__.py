@@ -1,0 +1,432 @@
+"""qrusty_amd -- qrusty's Pauli-operator path on AMD MI355X (gfx950).
+
+The Python surface keeps pyqrusty's names and behaviour for this path
+(pyqrusty/src/lib.rs:262-433, 476-491; pyqrusty/python/pyqrusty/__init__.py):
+
+    Pauli(label)                     .num_qubits() .label() repr/str .to_matrix()
+    SparsePauliOp(paulis, coeffs)    len, [i], [a:b:c], .num_qubits(), repr,
+                                     .to_matrix(), .to_matrix_mode(mode="")
+    SpMat                            .shape() .nnz() .diagonal() .export()
+                                     SpMat.new_unchecked(shape, data, indices, indptr)
+    spmat_dot_densevec(spmat, x)     y = A x (complex128)
+    csr_matrix(spmat)                scipy.sparse.csr_matrix from export()
+
+and adds the matrix-free path:
+
+    SparsePauliOp.apply(x, precision="fp64" | "mixed")
+    SparsePauliOp.expectation(psi, precision="fp64" | "mixed")
+
+Every compute call runs the HIP kernels of libqrhip.so through its C ABI
+(include/qrhip.h); there is no CPU fallback.  Matrices live in device memory
+(torch tensors are used only as device buffers and for the current stream).
+"""
+from __future__ import annotations
+
+import ctypes
+import decimal
+import math
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import QrError
+
+__all__ = ["Pauli", "SparsePauliOp", "SpMat", "spmat_dot_densevec", "csr_matrix", "QrError"]
+__version__ = "0.1.0"
+
+_PRECISIONS = {"fp64": L.QR_PREC_FP64, "mixed": L.QR_PREC_MIXED}
+
+
+# ---------------------------------------------------------------------------
+# device plumbing
+# ---------------------------------------------------------------------------
+def _torch():
+    import torch
+    return torch
+
+
+def _require_gpu():
+    torch = _torch()
+    if not torch.cuda.is_available():
+        raise QrError("qrusty_amd: no HIP device is available (the kernels have no CPU fallback)")
+    return torch
+
+
+def _stream_handle(torch):
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _rust_f64(v: float) -> str:
+    """Rust's Display for f64 (shortest round-trip digits, never an exponent),
+    used by util::complex64_to_string (qrusty/src/util.rs:140-142)."""
+    if math.isnan(v):
+        return "NaN"
+    if math.isinf(v):
+        return "inf" if v > 0 else "-inf"
+    s = format(decimal.Decimal(repr(v)), "f")
+    if "." in s:
+        s = s.rstrip("0").rstrip(".")
+    return s
+
+
+# ---------------------------------------------------------------------------
+# Pauli
+# ---------------------------------------------------------------------------
+class Pauli:
+    """A Pauli string (pyqrusty/src/lib.rs:262-303; qrusty/src/lib.rs:117-267)."""
+
+    __slots__ = ("_full", "_paulis", "_nq", "_base_phase", "_phase", "_x", "_z")
+
+    def __init__(self, label: str):
+        if not isinstance(label, str):
+            raise TypeError("Pauli label must be a str")
+        nq, bp, ph = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        x, z = ctypes.c_uint64(), ctypes.c_uint64()
+        L.check(L.lib.qr_pauli_parse(label.encode(), ctypes.byref(nq), ctypes.byref(bp),
+                                     ctypes.byref(ph), ctypes.byref(x), ctypes.byref(z)))
+        self._full = label
+        self._nq = nq.value
+        self._paulis = label[len(label) - nq.value:]
+        self._base_phase = bp.value
+        self._phase = ph.value
+        self._x = x.value
+        self._z = z.value
+
+    def num_qubits(self) -> int:
+        return self._nq
+
+    def label(self) -> str:
+        """The I/X/Y/Z characters only, without a phase prefix (lib.rs:150-154)."""
+        return self._paulis
+
+    def __repr__(self) -> str:
+        return f"Pauli('{self._paulis}')"
+
+    def __str__(self) -> str:
+        return self._paulis
+
+    def to_matrix(self) -> "SpMat":
+        """CSR of this Pauli string, i^base_phase times the Kronecker product
+        (lib.rs:203-213), built by the HIP rowwise kernel."""
+        return SparsePauliOp([self], [1.0 + 0.0j])._build(L.QR_PHASE_KRON)
+
+    # internal accessors (masks as in lib.rs:161-181)
+    @property
+    def x_mask(self) -> int:
+        return self._x
+
+    @property
+    def z_mask(self) -> int:
+        return self._z
+
+    @property
+    def phase(self) -> int:
+        return self._phase
+
+
+# ---------------------------------------------------------------------------
+# device operator handle
+# ---------------------------------------------------------------------------
+class _OpHandle:
+    def __init__(self, labels, coeffs, convention):
+        arr = (ctypes.c_char_p * len(labels))(*[s.encode() for s in labels])
+        c = np.ascontiguousarray(np.asarray(coeffs, dtype=np.complex128)).view(np.float64)
+        h = ctypes.c_void_p()
+        L.check(L.lib.qr_op_create(arr, c.ctypes.data_as(L.f64p), len(labels), convention,
+                                   ctypes.byref(h)))
+        self.h = h
+        nq, T, G, nnz = ctypes.c_uint32(), ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        L.check(L.lib.qr_op_info(h, ctypes.byref(nq), ctypes.byref(T), ctypes.byref(G),
+                                 ctypes.byref(nnz)))
+        self.num_qubits, self.T, self.G, self.nnz = nq.value, T.value, G.value, nnz.value
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value and L is not None and L.lib is not None:
+            L.lib.qr_op_destroy(h)
+            self.h = None
+
+
+# ---------------------------------------------------------------------------
+# SparsePauliOp
+# ---------------------------------------------------------------------------
+class SparsePauliOp:
+    """Sum of (Pauli, complex) terms (pyqrusty/src/lib.rs:305-433;
+    qrusty/src/lib.rs:345-593)."""
+
+    def __init__(self, paulis, coeffs):
+        paulis = list(paulis)
+        coeffs = [complex(c) for c in coeffs]
+        if len(paulis) != len(coeffs):
+            raise QrError("SparsePauliOp::new: paulis and coeffs must have same length")
+        if len(paulis) == 0:
+            raise QrError("SparsePauliOp::new: at least one pauli must be supplied")
+        for p in paulis:
+            if not isinstance(p, Pauli):
+                raise TypeError("SparsePauliOp: paulis must be Pauli objects")
+        nq = paulis[0].num_qubits()
+        if any(p.num_qubits() != nq for p in paulis):
+            raise QrError("SparsePauliOp::new: all supplied paulis must have the same #qubits")
+        self._members = list(zip(paulis, coeffs))
+        self._handles = {}
+
+    # -- reference API ------------------------------------------------------
+    def __len__(self) -> int:
+        return len(self._members)
+
+    def __getitem__(self, idx):
+        if isinstance(idx, slice):
+            return [m for m in self._members[idx]]
+        if not isinstance(idx, int):
+            raise TypeError("SparsePauliOp indices must be int or slice")
+        if not (0 <= idx < len(self._members)):
+            raise QrError(f"__getitem__ called on invalid index {idx}")
+        return self._members[idx]
+
+    def num_qubits(self) -> int:
+        return self._members[0][0].num_qubits()
+
+    def __repr__(self) -> str:
+        labels = "','".join(p.label() for p, _ in self._members)
+        coeffs = ", ".join(f"{_rust_f64(c.real)}+{_rust_f64(c.imag)}j" for _, c in self._members)
+        return f"SparsePauliOp('{labels}', [{coeffs}])"
+
+    __str__ = __repr__
+
+    def __add__(self, other: "SparsePauliOp") -> "SparsePauliOp":
+        """Concatenation of the term lists (qrusty/src/lib.rs:588-593)."""
+        out = SparsePauliOp.__new__(SparsePauliOp)
+        out._members = list(self._members) + list(other._members)
+        out._handles = {}
+        return out
+
+    def to_matrix(self) -> "SpMat":
+        return self.to_matrix_mode("")
+
+    def to_matrix_mode(self, mode: str = "") -> "SpMat":
+        """CSR of the operator (pyqrusty/src/lib.rs:406-415).
+
+        Every reference mode string is accepted (lib.rs:293-331) and served by
+        the HIP rowwise kernel; the mode only selects the reference's phase
+        convention for i/j-prefixed labels ("", Binary, Reduce, Rayon use
+        Pauli::to_matrix's i^base_phase; the others the rowwise (-i)^base_phase).
+        """
+        kind, step, conv = ctypes.c_int32(), ctypes.c_uint64(), ctypes.c_int32()
+        L.check(L.lib.qr_mode_parse(mode.encode(), ctypes.byref(kind), ctypes.byref(step),
+                                    ctypes.byref(conv)))
+        return self._build(conv.value)
+
+    # -- internals ----------------------------------------------------------
+    def _handle(self, convention: int) -> _OpHandle:
+        h = self._handles.get(convention)
+        if h is None:
+            labels = [p._full for p, _ in self._members]
+            coeffs = [c for _, c in self._members]
+            h = _OpHandle(labels, coeffs, convention)
+            self._handles[convention] = h
+        return h
+
+    def _build(self, convention: int) -> "SpMat":
+        torch = _require_gpu()
+        h = self._handle(convention)
+        N = 1 << h.num_qubits
+        if h.nnz == 0:
+            raise QrError("CSR too large: 2^n * G overflows 64 bits")
+        dev = torch.device("cuda", torch.cuda.current_device())
+        indptr = torch.empty(N + 1, dtype=torch.int64, device=dev)
+        indices = torch.empty(h.nnz, dtype=torch.int64, device=dev)
+        data = torch.empty(h.nnz, dtype=torch.complex128, device=dev)
+        L.check(L.lib.qr_csr_build(h.h, _ptr(indptr), _ptr(indices), _ptr(data),
+                                   _stream_handle(torch)))
+        return SpMat._from_device((N, N), indptr, indices, data)
+
+    # -- matrix-free path (no reference counterpart) --------------------------
+    def apply(self, x, precision: str = "fp64", out=None):
+        """y = H x without building the matrix.
+
+        x: (N,) or (nvec, N) states -- numpy complex128, or torch device tensors
+        (complex128 for "fp64", complex64 for "mixed").  Returns the same kind
+        of object; a numpy input gives a numpy complex128 result.
+        """
+        return self._apply(x, precision, out=out, want_y=True, want_e=False)[0]
+
+    def expectation(self, psi, precision: str = "fp64"):
+        """<psi|H|psi> per state, reduced in FP64 on the device (y is not stored)."""
+        return self._apply(psi, precision, out=None, want_y=False, want_e=True)[1]
+
+    def apply_and_expectation(self, x, precision: str = "fp64", out=None):
+        """(y = H x, <x|H|x>) from one pass."""
+        return self._apply(x, precision, out=out, want_y=True, want_e=True)
+
+    def _apply(self, x, precision, out, want_y, want_e):
+        if precision not in _PRECISIONS:
+            raise QrError(f"unknown precision {precision!r} (expected 'fp64' or 'mixed')")
+        torch = _require_gpu()
+        h = self._handle(L.QR_PHASE_ROWWISE)
+        N = 1 << h.num_qubits
+        prec = _PRECISIONS[precision]
+        dt = torch.complex64 if prec == L.QR_PREC_MIXED else torch.complex128
+        is_np = not isinstance(x, torch.Tensor)
+        if is_np:
+            xa = np.asarray(x)
+            squeeze = xa.ndim == 1
+            xt = torch.from_numpy(np.ascontiguousarray(xa.astype(np.complex128, copy=False)))
+            xt = xt.to(device="cuda").to(dt)
+        else:
+            squeeze = x.dim() == 1
+            if x.dtype != dt or not x.is_cuda:
+                raise QrError(f"apply: expected a device tensor of dtype {dt}")
+            xt = x.contiguous()
+        if xt.dim() == 1:
+            xt = xt.reshape(1, -1)
+        if xt.dim() != 2 or xt.shape[1] != N:
+            raise QrError(f"apply: state length must be 2^{h.num_qubits} = {N}")
+        nvec = xt.shape[0]
+        y = None
+        if want_y:
+            y = out if out is not None else torch.empty_like(xt)
+            if y.shape != xt.shape or y.dtype != dt or not y.is_contiguous():
+                raise QrError("apply: out has the wrong shape, dtype or layout")
+        e = torch.empty(2 * nvec, dtype=torch.float64, device=xt.device) if want_e else None
+        L.check(L.lib.qr_pauli_apply(h.h, prec, nvec, _ptr(xt),
+                                     _ptr(y) if y is not None else None,
+                                     _ptr(e) if e is not None else None, _stream_handle(torch)))
+        ev = None
+        if e is not None:
+            ev = e.cpu().numpy().view(np.complex128)
+            if squeeze:
+                ev = ev[0]
+        if y is not None:
+            if squeeze:
+                y = y.reshape(-1)
+            if is_np:
+                y = y.to(torch.complex128).cpu().numpy()
+        return y, ev
+
+
+# ---------------------------------------------------------------------------
+# SpMat
+# ---------------------------------------------------------------------------
+class SpMat:
+    """CSR matrix with uint64 indptr/indices and complex128 data held in device
+    memory (pyqrusty/src/lib.rs:33-260).  export() moves it out, after which
+    the object is dead, like the reference's Box<Option<CsMatI>>."""
+
+    def __init__(self):
+        raise TypeError("SpMat objects come from to_matrix_mode() or SpMat.new_unchecked()")
+
+    @classmethod
+    def _from_device(cls, shape, indptr, indices, data) -> "SpMat":
+        m = cls.__new__(cls)
+        m._shape = (int(shape[0]), int(shape[1]))
+        m._indptr, m._indices, m._data = indptr, indices, data
+        return m
+
+    @staticmethod
+    def new_unchecked(shape, data, indices, indptr) -> "SpMat":
+        """Wraps CSR arrays without validation (pyqrusty/src/lib.rs:103-116)."""
+        torch = _require_gpu()
+        d = np.ascontiguousarray(np.asarray(data, dtype=np.complex128))
+        i = np.ascontiguousarray(np.asarray(indices).astype(np.uint64)).view(np.int64)
+        p = np.ascontiguousarray(np.asarray(indptr).astype(np.uint64)).view(np.int64)
+        dev = torch.device("cuda", torch.cuda.current_device())
+        return SpMat._from_device(tuple(shape), torch.from_numpy(p.copy()).to(dev),
+                                  torch.from_numpy(i.copy()).to(dev),
+                                  torch.from_numpy(d.copy()).to(dev))
+
+    def _live(self, msg: str):
+        if self._data is None:
+            raise QrError(msg)
+
+    def shape(self):
+        self._live("SpMat.shape(): matrix is already dropped")
+        return self._shape
+
+    def nnz(self) -> int:
+        self._live("cannot get NNZ of an exported sparse matrix")
+        return int(self._data.numel())
+
+    def diagonal(self):
+        """Dense main diagonal (pyqrusty/src/lib.rs:118-125), computed from a host copy."""
+        self._live("SpMat.diagonal(): already-exported sparse matrix")
+        indptr = self._indptr.cpu().numpy().view(np.uint64)
+        indices = self._indices.cpu().numpy().view(np.uint64)
+        data = self._data.cpu().numpy()
+        n = min(self._shape)
+        out = np.zeros(n, dtype=np.complex128)
+        rows = np.repeat(np.arange(self._shape[0], dtype=np.uint64), np.diff(indptr).astype(np.int64))
+        hit = (rows == indices) & (rows < n)
+        out[rows[hit].astype(np.int64)] = data[hit]
+        return out
+
+    def export(self):
+        """((rows, cols), data complex128, indices uint64, indptr uint64); the
+        SpMat is dead afterwards (pyqrusty/src/lib.rs:190-214)."""
+        self._live("cannot export from an already-exported sparse matrix")
+        data = self._data.cpu().numpy()
+        indices = self._indices.cpu().numpy().view(np.uint64)
+        indptr = self._indptr.cpu().numpy().view(np.uint64)
+        shape = self._shape
+        self._indptr = self._indices = self._data = None
+        return shape, data, indices, indptr
+
+    def device_arrays(self):
+        """(indptr, indices, data) device tensors (int64 views of uint64 values)."""
+        self._live("cannot access an exported sparse matrix")
+        return self._indptr, self._indices, self._data
+
+    def __copy__(self) -> "SpMat":
+        if self._data is None:
+            m = SpMat.__new__(SpMat)
+            m._shape, m._indptr, m._indices, m._data = self._shape, None, None, None
+            return m
+        return SpMat._from_device(self._shape, self._indptr.clone(), self._indices.clone(),
+                                  self._data.clone())
+
+    def __repr__(self) -> str:
+        if self._data is None:
+            return "<already-dropped sparse matrix of type Complex64>"
+        r, c = self._shape
+        return (f"<{r}x{c} sparse matrix of type Complex64\n\twith {self.nnz()} stored "
+                "elements in Compressed Sparse Row format>")
+
+    __str__ = __repr__
+
+
+def spmat_dot_densevec(spmat: SpMat, x):
+    """y = spmat @ x on the device (pyqrusty/src/lib.rs:476-491; accel.rs:338-370).
+
+    x: 1-D numpy complex128 (returns numpy) or a complex128 device tensor
+    (returns a device tensor on the current stream)."""
+    if spmat._data is None:
+        raise QrError("cannot multiply with an exported sparse matrix")
+    torch = _require_gpu()
+    rows, cols = spmat._shape
+    is_np = not isinstance(x, torch.Tensor)
+    if is_np:
+        xa = np.ascontiguousarray(np.asarray(x, dtype=np.complex128))
+        if xa.ndim != 1:
+            raise QrError("spmat_dot_densevec: x must be one-dimensional")
+        xt = torch.from_numpy(xa).to(spmat._data.device)
+    else:
+        if x.dtype != torch.complex128 or x.dim() != 1 or not x.is_cuda:
+            raise QrError("spmat_dot_densevec: x must be a 1-D complex128 device tensor")
+        xt = x.contiguous()
+    if xt.numel() != cols:
+        raise QrError(f"spmat_dot_densevec: dimension mismatch ({xt.numel()} != {cols})")
+    y = torch.empty(rows, dtype=torch.complex128, device=xt.device)
+    L.check(L.lib.qr_csr_matvec(rows, cols, spmat._data.numel(), _ptr(spmat._indptr),
+                                _ptr(spmat._indices), _ptr(spmat._data), _ptr(xt), _ptr(y),
+                                _stream_handle(torch)))
+    return y.cpu().numpy() if is_np else y
+
+
+def csr_matrix(m: SpMat):
+    """scipy.sparse.csr_matrix of an SpMat (pyqrusty/python/pyqrusty/__init__.py:14-17)."""
+    (shape, data, indices, indptr) = m.export()
+    from scipy.sparse import csr_matrix as _csr
+    return _csr((data, indices, indptr), shape=shape, dtype=complex)

@@ -1,0 +1,78 @@
+"""ctypes binding of libqrhip.so (the C ABI declared in include/qrhip.h).
+
+The library is loaded eagerly; a missing or unloadable library raises
+ImportError, so the package never runs without its HIP kernels.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libqrhip.so")
+
+QR_OK = 0
+QR_PHASE_ROWWISE = 0
+QR_PHASE_KRON = 1
+QR_PREC_FP64 = 0
+QR_PREC_MIXED = 1
+
+u64 = ctypes.c_uint64
+u32 = ctypes.c_uint32
+i32 = ctypes.c_int32
+vp = ctypes.c_void_p
+u64p = ctypes.POINTER(ctypes.c_uint64)
+u32p = ctypes.POINTER(ctypes.c_uint32)
+i32p = ctypes.POINTER(ctypes.c_int32)
+f64p = ctypes.POINTER(ctypes.c_double)
+
+# name -> (restype, argtypes); mirrors include/qrhip.h
+SIGNATURES = {
+    "qr_last_error": (ctypes.c_char_p, []),
+    "qr_abi_version": (ctypes.c_int, []),
+    "qr_pauli_parse": (ctypes.c_int, [ctypes.c_char_p, u32p, u32p, u32p, u64p, u64p]),
+    "qr_op_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_char_p), f64p, u64, i32,
+                                    ctypes.POINTER(vp)]),
+    "qr_op_destroy": (ctypes.c_int, [vp]),
+    "qr_op_info": (ctypes.c_int, [vp, u32p, u64p, u64p, u64p]),
+    "qr_op_terms": (ctypes.c_int, [vp, u64p, u64p, f64p]),
+    "qr_op_row_layout": (ctypes.c_int, [vp, u64, u64p, u32p]),
+    "qr_mode_parse": (ctypes.c_int, [ctypes.c_char_p, i32p, u64p, i32p]),
+    "qr_csr_build": (ctypes.c_int, [vp, vp, vp, vp, vp]),
+    "qr_csr_build_rows": (ctypes.c_int, [vp, u64, u64, vp, vp, vp]),
+    "qr_csr_matvec": (ctypes.c_int, [u64, u64, u64, vp, vp, vp, vp, vp, vp]),
+    "qr_pauli_apply": (ctypes.c_int, [vp, i32, u32, vp, vp, vp, vp]),
+}
+
+
+class QrError(Exception):
+    """Raised for every failure reported by libqrhip (the reference raises
+    PyException from QrustyErr, qrusty/src/lib.rs:48-52)."""
+
+    def __init__(self, message: str, code: int = -1):
+        super().__init__(message)
+        self.code = code
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"qrusty_amd: {LIB_PATH} is not built; run `python -m qrusty_amd.build` "
+            "(the HIP kernels have no CPU fallback)")
+    try:
+        L = ctypes.CDLL(LIB_PATH)
+    except OSError as e:  # pragma: no cover - depends on the ROCm install
+        raise ImportError(f"qrusty_amd: cannot load {LIB_PATH}: {e}") from e
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    return L
+
+
+lib = _load()
+
+
+def check(rc: int) -> None:
+    if rc != QR_OK:
+        raise QrError(lib.qr_last_error().decode(errors="replace"), rc)

@@ -1,0 +1,376 @@
+// qr_api.cpp -- host side of libqrhip: labels, operators, modes, C ABI glue.
+//
+// Mirrors the reference's host logic (qrusty/src/lib.rs:117-585 and
+// accel.rs:141-157) and prepares the device tables the HIP kernels read:
+//   * terms grouped by x mask (a stable sort, so a group's terms keep their
+//     input order, which is the order make_row folds them in, accel.rs:188-208);
+//   * per-group "rank lists" that give each group's position in the ascending
+//     column order of any row r without sorting (see build_rank_lists).
+#include "qr_internal.h"
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+
+namespace qr {
+
+static thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+    return fail(QR_ERR_HIP, std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
+}
+
+// Label grammar of qrusty/src/lib.rs:127: ^([+-]?)1?([ij]?)([IXYZ]+)$.
+// Qubit q is label[len-1-q] (lib.rs:144); x bit for X|Y and z bit for Y|Z
+// (lib.rs:80-91); phase = (base_phase + #Y) mod 4 (lib.rs:179-181).
+static int parse_label(const char *s, uint32_t *nq, uint32_t *base_phase, uint32_t *ny,
+                       uint64_t *x, uint64_t *z) {
+    if (!s) return fail(QR_ERR_ARG, "null label");
+    const char *p = s;
+    bool neg = false, imag = false;
+    if (*p == '+' || *p == '-') { neg = (*p == '-'); ++p; }
+    if (*p == '1') ++p;
+    if (*p == 'i' || *p == 'j') { imag = true; ++p; }
+    size_t n = std::strlen(p);
+    if (n == 0) return fail(QR_ERR_LABEL, "error: malformed label");
+    for (size_t k = 0; k < n; ++k)
+        if (p[k] != 'I' && p[k] != 'X' && p[k] != 'Y' && p[k] != 'Z')
+            return fail(QR_ERR_LABEL, "error: malformed label");
+    if (n > 63) return fail(QR_ERR_SIZE, "Pauli labels of more than 63 qubits are not supported");
+    uint64_t xm = 0, zm = 0;
+    uint32_t y = 0;
+    for (size_t q = 0; q < n; ++q) {
+        char c = p[n - 1 - q];
+        if (c == 'X' || c == 'Y') xm |= uint64_t(1) << q;
+        if (c == 'Y' || c == 'Z') zm |= uint64_t(1) << q;
+        if (c == 'Y') ++y;
+    }
+    *nq = uint32_t(n);
+    *base_phase = (imag ? 1u : 0u) + (neg ? 2u : 0u);
+    *ny = y;
+    *x = xm;
+    *z = zm;
+    return QR_OK;
+}
+
+// Rank lists.  Sort the G distinct masks ascending (the column order of row 0)
+// and view them as a binary trie, splitting at the highest differing bit.
+// Row r orders its columns r ^ x ascending, which is the same trie with the
+// two children of every node splitting at bit b swapped when bit b of r is
+// set.  A group at sorted position j therefore sits at
+//     rank_r(j) = j + sum over the splitting nodes on its path with r_b = 1 of
+//                 (+size of right child if j is in the left child,
+//                  -size of left child  if j is in the right child).
+// Each (bit, sigma) pair is one RankEntry; a path has at most min(n, G-1) of them.
+static void build_rank_lists(const std::vector<uint64_t> &gx, std::vector<uint32_t> &ptr,
+                             std::vector<RankEntry> &out) {
+    const size_t G = gx.size();
+    std::vector<std::vector<RankEntry>> lists(G);
+    struct Range { size_t lo, hi; };
+    std::vector<Range> stack;
+    stack.push_back({0, G});
+    while (!stack.empty()) {
+        Range rg = stack.back();
+        stack.pop_back();
+        if (rg.hi - rg.lo <= 1) continue;
+        uint64_t diff = gx[rg.lo] ^ gx[rg.hi - 1];
+        int b = 63 - __builtin_clzll(diff);
+        uint64_t bit = uint64_t(1) << b;
+        // sorted and sharing all bits above b: first index with bit b set
+        size_t m = size_t(std::partition_point(gx.begin() + rg.lo, gx.begin() + rg.hi,
+                                               [bit](uint64_t v) { return (v & bit) == 0; }) -
+                          gx.begin());
+        int32_t right = int32_t(rg.hi - m), left = int32_t(m - rg.lo);
+        for (size_t j = rg.lo; j < m; ++j) lists[j].push_back({b, right});
+        for (size_t j = m; j < rg.hi; ++j) lists[j].push_back({b, -left});
+        stack.push_back({rg.lo, m});
+        stack.push_back({m, rg.hi});
+    }
+    ptr.assign(G + 1, 0);
+    for (size_t j = 0; j < G; ++j) ptr[j + 1] = ptr[j] + uint32_t(lists[j].size());
+    out.clear();
+    out.reserve(ptr[G]);
+    for (size_t j = 0; j < G; ++j) out.insert(out.end(), lists[j].begin(), lists[j].end());
+}
+
+static void free_tables(DeviceTables &t) {
+    void *ptrs[] = {t.gx, t.gterm, t.tz, t.tc, t.rl_ptr, t.rl, t.gkind, t.partials};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    t = DeviceTables{};
+}
+
+template <typename T>
+static int upload(T **dst, const std::vector<T> &src) {
+    size_t bytes = std::max<size_t>(src.size(), 1) * sizeof(T);
+    QR_HIP_CHECK(hipMalloc((void **)dst, bytes));
+    if (!src.empty()) QR_HIP_CHECK(hipMemcpy(*dst, src.data(), src.size() * sizeof(T),
+                                             hipMemcpyHostToDevice));
+    return QR_OK;
+}
+
+int device_tables(const qr_op *op, DeviceTables **out) {
+    int dev = 0;
+    QR_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(QR_ERR_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lock(op->mu);
+    DeviceTables &t = op->dev[dev];
+    if (!t.ready) {
+        int rc;
+        std::vector<double2> tc2(op->T);
+        for (uint64_t i = 0; i < op->T; ++i) tc2[i] = make_double2(op->tc[2 * i], op->tc[2 * i + 1]);
+        if ((rc = upload(&t.gx, op->gx)) || (rc = upload(&t.gterm, op->gterm)) ||
+            (rc = upload(&t.tz, op->tz)) || (rc = upload(&t.tc, tc2)) ||
+            (rc = upload(&t.rl_ptr, op->rl_ptr)) || (rc = upload(&t.rl, op->rl)) ||
+            (rc = upload(&t.gkind, op->gkind))) {
+            free_tables(t);
+            return rc;
+        }
+        t.ready = true;
+    }
+    *out = &t;
+    return QR_OK;
+}
+
+}  // namespace qr
+
+using namespace qr;
+
+extern "C" {
+
+const char *qr_last_error(void) { return g_last_error.c_str(); }
+
+int qr_abi_version(void) { return QRHIP_ABI_VERSION; }
+
+int qr_pauli_parse(const char *label, uint32_t *num_qubits, uint32_t *base_phase,
+                   uint32_t *phase, uint64_t *x_mask, uint64_t *z_mask) {
+    uint32_t nq, bp, ny;
+    uint64_t x, z;
+    int rc = parse_label(label, &nq, &bp, &ny, &x, &z);
+    if (rc) return rc;
+    if (num_qubits) *num_qubits = nq;
+    if (base_phase) *base_phase = bp;
+    if (phase) *phase = (bp + ny) % 4;
+    if (x_mask) *x_mask = x;
+    if (z_mask) *z_mask = z;
+    return QR_OK;
+}
+
+int qr_op_create(const char *const *labels, const double *coeffs, uint64_t num_terms,
+                 int32_t phase_convention, qr_op **out) {
+#pragma clang fp contract(off)
+    if (!out) return fail(QR_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (num_terms == 0)
+        return fail(QR_ERR_TERMS, "SparsePauliOp::new: at least one pauli must be supplied");
+    if (!labels || !coeffs) return fail(QR_ERR_ARG, "null labels or coefficients");
+    if (phase_convention != QR_PHASE_ROWWISE && phase_convention != QR_PHASE_KRON)
+        return fail(QR_ERR_ARG, "unknown phase convention");
+    if (num_terms > 0xFFFFFFF0ull) return fail(QR_ERR_SIZE, "too many terms");
+    qr_op *op = new (std::nothrow) qr_op;
+    if (!op) return fail(QR_ERR_NOMEM, "out of host memory");
+    const uint64_t T = num_terms;
+    op->T = T;
+    op->x.resize(T);
+    op->z.resize(T);
+    op->cp.resize(2 * T);
+    // M[p] of accel.rs:147-152, multiplied as num-complex does:
+    // (a.re*b.re - a.im*b.im, a.re*b.im + a.im*b.re) with a = M[p], b = c.
+    static const double M[4][2] = {{1.0, 0.0}, {0.0, -1.0}, {-1.0, 0.0}, {0.0, 1.0}};
+    for (uint64_t t = 0; t < T; ++t) {
+        uint32_t nq, bp, ny;
+        int rc = parse_label(labels[t], &nq, &bp, &ny, &op->x[t], &op->z[t]);
+        if (rc) { delete op; return rc; }
+        if (t == 0) op->nq = nq;
+        else if (nq != op->nq) {
+            delete op;
+            return fail(QR_ERR_TERMS,
+                        "SparsePauliOp::new: all supplied paulis must have the same #qubits");
+        }
+        uint32_t p = phase_convention == QR_PHASE_ROWWISE ? (bp + ny) % 4 : (ny + 4 - bp) % 4;
+        op->cp[2 * t] = M[p][0] * coeffs[2 * t] - M[p][1] * coeffs[2 * t + 1];
+        op->cp[2 * t + 1] = M[p][0] * coeffs[2 * t + 1] + M[p][1] * coeffs[2 * t];
+    }
+    // group by x: stable sort keeps input order inside a group
+    std::vector<uint32_t> order(T);
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(),
+                     [op](uint32_t a, uint32_t b) { return op->x[a] < op->x[b]; });
+    op->tz.resize(T);
+    op->tc.resize(2 * T);
+    for (uint64_t i = 0; i < T; ++i) {
+        uint32_t t = order[i];
+        if (i == 0 || op->x[t] != op->gx.back()) {
+            op->gx.push_back(op->x[t]);
+            op->gterm.push_back(uint32_t(i));
+        }
+        op->tz[i] = op->z[t];
+        op->tc[2 * i] = op->cp[2 * t];
+        op->tc[2 * i + 1] = op->cp[2 * t + 1];
+    }
+    op->gterm.push_back(uint32_t(T));
+    op->G = op->gx.size();
+    op->gkind.resize(op->G);
+    for (uint64_t g = 0; g < op->G; ++g) {
+        bool real = true, imag = true;
+        for (uint32_t i = op->gterm[g]; i < op->gterm[g + 1]; ++i) {
+            if (op->tc[2 * i + 1] != 0.0) real = false;
+            if (op->tc[2 * i] != 0.0) imag = false;
+        }
+        op->gkind[g] = real ? 1 : (imag ? 2 : 0);
+    }
+    build_rank_lists(op->gx, op->rl_ptr, op->rl);
+    *out = op;
+    return QR_OK;
+}
+
+int qr_op_destroy(qr_op *op) {
+    if (!op) return QR_OK;
+    int cur = 0;
+    bool have = hipGetDevice(&cur) == hipSuccess;
+    for (int d = 0; d < kMaxDevices; ++d) {
+        if (!op->dev[d].ready && !op->dev[d].gx) continue;
+        if (hipSetDevice(d) == hipSuccess) free_tables(op->dev[d]);
+    }
+    if (have) (void)hipSetDevice(cur);
+    delete op;
+    return QR_OK;
+}
+
+int qr_op_info(const qr_op *op, uint32_t *num_qubits, uint64_t *num_terms, uint64_t *num_groups,
+               uint64_t *nnz) {
+    if (!op) return fail(QR_ERR_ARG, "null operator");
+    if (num_qubits) *num_qubits = op->nq;
+    if (num_terms) *num_terms = op->T;
+    if (num_groups) *num_groups = op->G;
+    if (nnz) {
+        unsigned __int128 v = (unsigned __int128)op->G << op->nq;
+        *nnz = (v >> 64) ? 0 : uint64_t(v);
+    }
+    return QR_OK;
+}
+
+int qr_op_terms(const qr_op *op, uint64_t *x, uint64_t *z, double *coeff) {
+    if (!op) return fail(QR_ERR_ARG, "null operator");
+    if (x) std::memcpy(x, op->x.data(), op->T * sizeof(uint64_t));
+    if (z) std::memcpy(z, op->z.data(), op->T * sizeof(uint64_t));
+    if (coeff) std::memcpy(coeff, op->cp.data(), 2 * op->T * sizeof(double));
+    return QR_OK;
+}
+
+int qr_op_row_layout(const qr_op *op, uint64_t row, uint64_t *cols, uint32_t *groups) {
+    if (!op) return fail(QR_ERR_ARG, "null operator");
+    if (op->nq < 64 && (row >> op->nq) != 0) return fail(QR_ERR_ARG, "row out of range");
+    for (uint64_t j = 0; j < op->G; ++j) {
+        int64_t rank = int64_t(j);
+        for (uint32_t e = op->rl_ptr[j]; e < op->rl_ptr[j + 1]; ++e)
+            if ((row >> op->rl[e].bit) & 1) rank += op->rl[e].sigma;
+        if (rank < 0 || uint64_t(rank) >= op->G)
+            return fail(QR_ERR_ARG, "internal error: rank out of range");
+        if (cols) cols[rank] = row ^ op->gx[j];
+        if (groups) groups[rank] = uint32_t(j);
+    }
+    return QR_OK;
+}
+
+int qr_mode_parse(const char *mode, int32_t *kind, uint64_t *step, int32_t *phase_convention) {
+    if (!mode) return fail(QR_ERR_ARG, "null mode");
+    std::string m(mode);
+    int32_t k = -1;
+    uint64_t st = 0;
+    auto find_step = [&](const char *key) -> int {
+        // unanchored search like Regex::is_match(r"<key>/(\d+)") (lib.rs:283-288)
+        size_t klen = std::strlen(key);
+        for (size_t pos = m.find(key); pos != std::string::npos; pos = m.find(key, pos + 1)) {
+            size_t d = pos + klen;
+            if (d < m.size() && m[d] == '/' && d + 1 < m.size() && m[d + 1] >= '0' && m[d + 1] <= '9') {
+                uint64_t v = 0;
+                bool overflow = false;
+                for (size_t i = d + 1; i < m.size() && m[i] >= '0' && m[i] <= '9'; ++i) {
+                    if (v > (UINT64_MAX - 9) / 10) overflow = true;
+                    v = v * 10 + uint64_t(m[i] - '0');
+                }
+                st = overflow ? 0 : v;
+                return 1;
+            }
+        }
+        return 0;
+    };
+    if (m.empty()) k = QR_MODE_DEFAULT;
+    else if (m == "Binary") k = QR_MODE_BINARY;
+    else if (m == "Accel") k = QR_MODE_ACCEL;
+    else if (m == "Rowwise") k = QR_MODE_ROWWISE;
+    else if (m == "RowwiseUnsafe") k = QR_MODE_ROWWISE_UNSAFE;
+    else if (find_step("RowwiseUnsafeChunked")) k = QR_MODE_ROWWISE_UNSAFE_CHUNKED;
+    else if (m == "Reduce") k = QR_MODE_REDUCE;
+    else if (m == "Rayon") k = QR_MODE_RAYON;
+    else if (find_step("RayonChunked")) k = QR_MODE_RAYON_CHUNKED;
+    else if (m == "Hip") k = QR_MODE_HIP;
+    if (k < 0) return fail(QR_ERR_MODE, "to_matrix_mode: unrecognized mode " + m);
+    if ((k == QR_MODE_ROWWISE_UNSAFE_CHUNKED || k == QR_MODE_RAYON_CHUNKED) && st == 0)
+        return fail(QR_ERR_MODE, "to_matrix_mode: chunk size must be a positive integer in " + m);
+    if (kind) *kind = k;
+    if (step) *step = st;
+    if (phase_convention) {
+        bool kron = (k == QR_MODE_DEFAULT || k == QR_MODE_BINARY || k == QR_MODE_REDUCE ||
+                     k == QR_MODE_RAYON);
+        *phase_convention = kron ? QR_PHASE_KRON : QR_PHASE_ROWWISE;
+    }
+    return QR_OK;
+}
+
+int qr_csr_build(const qr_op *op, uint64_t *d_indptr, uint64_t *d_indices, double *d_data,
+                 void *stream) {
+    if (!op || !d_indptr || !d_indices || !d_data) return fail(QR_ERR_ARG, "null argument");
+    if (op->nq > 40) return fail(QR_ERR_SIZE, "CSR of more than 40 qubits is not supported");
+    DeviceTables *t;
+    int rc = device_tables(op, &t);
+    if (rc) return rc;
+    return launch_csr_build(op, t, 0, uint64_t(1) << op->nq, d_indptr, d_indices, d_data,
+                            (hipStream_t)stream);
+}
+
+int qr_csr_build_rows(const qr_op *op, uint64_t row_begin, uint64_t row_end, uint64_t *d_indices,
+                      double *d_data, void *stream) {
+    if (!op || !d_indices || !d_data) return fail(QR_ERR_ARG, "null argument");
+    if (op->nq > 40) return fail(QR_ERR_SIZE, "CSR of more than 40 qubits is not supported");
+    if (row_end < row_begin || row_end > (uint64_t(1) << op->nq))
+        return fail(QR_ERR_ARG, "row range out of bounds");
+    if (row_end == row_begin) return QR_OK;
+    DeviceTables *t;
+    int rc = device_tables(op, &t);
+    if (rc) return rc;
+    return launch_csr_build(op, t, row_begin, row_end, nullptr, d_indices, d_data,
+                            (hipStream_t)stream);
+}
+
+int qr_csr_matvec(uint64_t nrows, uint64_t ncols, uint64_t nnz, const uint64_t *d_indptr,
+                  const uint64_t *d_indices, const double *d_data, const double *d_x, double *d_y,
+                  void *stream) {
+    if (nrows == 0) return QR_OK;
+    if (!d_indptr || !d_y) return fail(QR_ERR_ARG, "null argument");
+    if (nnz != 0 && (!d_indices || !d_data || !d_x || ncols == 0))
+        return fail(QR_ERR_ARG, "null argument");
+    return launch_csr_matvec(nrows, ncols, nnz, d_indptr, d_indices, d_data, d_x, d_y,
+                             (hipStream_t)stream);
+}
+
+int qr_pauli_apply(const qr_op *op, int32_t precision, uint32_t nvec, const void *d_x, void *d_y,
+                   double *d_expect, void *stream) {
+    if (!op || !d_x || (!d_y && !d_expect)) return fail(QR_ERR_ARG, "null argument");
+    if (precision != QR_PREC_FP64 && precision != QR_PREC_MIXED)
+        return fail(QR_ERR_ARG, "unknown precision");
+    if (nvec == 0) return QR_OK;
+    if (op->nq > 36) return fail(QR_ERR_SIZE, "states of more than 36 qubits are not supported");
+    DeviceTables *t;
+    int rc = device_tables(op, &t);
+    if (rc) return rc;
+    return launch_pauli_apply(op, t, precision, nvec, d_x, d_y, d_expect, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,261 @@
+"""GPU parity of the HIP CSR build and CSR matvec against the oracle.
+
+Bar: indptr/indices/data bit-exact (SHA-256 of the exported bytes, or byte
+equality with the oracle's rowwise build); matvec within 1e-12 relative
+(normwise, and componentwise against sum_k |a_k||x_k|).  Also the reference's
+own Python tests for this path (pyqrusty/tests/test_H.py, test_it.py:28-230),
+rewritten against this package.
+"""
+import hashlib
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def _op(labels, coeffs):
+    import qrusty_amd as qa
+    return qa.SparsePauliOp([qa.Pauli(l) for l in labels], coeffs)
+
+
+def _export(spmat):
+    shape, data, indices, indptr = spmat.export()
+    return shape, data, indices, indptr
+
+
+def _check_matvec(y, ref, A_abs_x):
+    err = np.abs(y - ref)
+    assert np.linalg.norm(y - ref) <= 1e-12 * np.linalg.norm(ref)
+    assert np.all(err <= 1e-12 * A_abs_x + 1e-300)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import qrusty_amd  # noqa: F401  (fails loudly if libqrhip.so is missing)
+    yield
+    torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("name", ["py_H2", "rs_H2", "py_H4", "rs_H4", "py_H6", "rs_H6", "py_H8"])
+def test_fixture_csr_bit_exact(hamiltonians, csr_digests, name):
+    print(f"[gpu] csr fixture {name}", flush=True)
+    labels, coeffs = hamiltonians[name]
+    m = _op(labels, coeffs).to_matrix_mode("RowwiseUnsafeChunked/100")
+    shape, data, indices, indptr = _export(m)
+    dg = csr_digests[name]
+    assert shape == (1 << dg["num_qubits"],) * 2
+    assert len(data) == dg["nnz"]
+    assert _sha(indptr) == dg["indptr_sha256"]
+    assert _sha(indices) == dg["indices_sha256"]
+    assert _sha(data) == dg["data_sha256"]
+
+
+def test_config1_build_and_matvec(config1):
+    print("[gpu] config1 12q/200", flush=True)
+    import qrusty_amd as qa
+    op = _op(config1["labels"], config1["coeffs"])
+    m = op.to_matrix_mode("RowwiseUnsafeChunked/100")
+    from qrusty_amd import workloads
+    x = workloads.random_states(1 << 12, 1, seed=workloads.SEED_CFG1 + 1)[0]
+    y = qa.spmat_dot_densevec(m, x)
+    shape, data, indices, indptr = _export(m)
+    assert _sha(indices) == config1["csr"]["indices_sha256"]
+    assert _sha(data) == config1["csr"]["data_sha256"]
+    assert _sha(indptr) == config1["csr"]["indptr_sha256"]
+    import scipy.sparse as sps
+    A = sps.csr_matrix((np.abs(data), indices.astype(np.int64), indptr.astype(np.int64)), shape=shape)
+    _check_matvec(y, config1["y"], A.dot(np.abs(x)))
+
+
+def _random_labels(rng, n, T, prefixes=True):
+    pre = ["", "+", "-", "i", "j", "-i", "+j", "1", "-1", "-1j"] if prefixes else [""]
+    labels = [str(rng.choice(pre)) + "".join(rng.choice(list("IXYZ"), size=n)) for _ in range(T)]
+    for _ in range(T // 4):
+        labels[rng.integers(0, T)] = labels[rng.integers(0, T)]
+    return labels
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_ops_bit_exact(seed):
+    rng = np.random.default_rng(1000 + seed)
+    n = int(rng.integers(1, 13))
+    T = int(rng.integers(1, 300))
+    labels = _random_labels(rng, n, T)
+    coeffs = rng.standard_normal(T) + 1j * rng.standard_normal(T)
+    coeffs[rng.random(T) < 0.1] = 0.0
+    print(f"[gpu] random op seed={seed} n={n} T={T}", flush=True)
+    p = orc.Params(labels, coeffs)
+    ip, ind, dat = orc.build_csr(p)
+    shape, data, indices, indptr = _export(_op(labels, coeffs).to_matrix_mode("Hip"))
+    assert indptr.tobytes() == ip.tobytes()
+    assert indices.tobytes() == ind.tobytes()
+    assert data.tobytes() == dat.tobytes()
+
+
+def test_kron_modes_match_dense():
+    """Modes "", Binary, Reduce, Rayon follow Pauli::to_matrix's i^base_phase
+    (lib.rs:182-213, 401-412)."""
+    rng = np.random.default_rng(9)
+    labels = _random_labels(rng, 5, 40)
+    coeffs = rng.standard_normal(40) + 1j * rng.standard_normal(40)
+    dense_k = orc.dense_sum(labels, coeffs, "kron")
+    dense_r = orc.dense_sum(labels, coeffs, "rowwise")
+    import qrusty_amd as qa
+    op = _op(labels, coeffs)
+    for mode, ref in [("", dense_k), ("Binary", dense_k), ("Reduce", dense_k), ("Rayon", dense_k),
+                      ("Accel", dense_r), ("Rowwise", dense_r), ("RowwiseUnsafe", dense_r),
+                      ("RayonChunked/7", dense_r), ("RowwiseUnsafeChunked/3", dense_r)]:
+        got = qa.csr_matrix(op.to_matrix_mode(mode)).toarray()
+        assert np.array_equal(got, ref), mode
+
+
+def test_build_rows_slices_and_wide_masks():
+    """qr_csr_build_rows on row slices, including 40-qubit operators (64-bit masks)."""
+    import ctypes
+    import qrusty_amd as qa
+    from qrusty_amd import _lib as L
+    rng = np.random.default_rng(21)
+    for n, T in [(10, 50), (33, 20), (40, 64)]:
+        labels = _random_labels(rng, n, T, prefixes=False)
+        coeffs = rng.standard_normal(T) + 1j * rng.standard_normal(T)
+        print(f"[gpu] build_rows n={n} T={T}", flush=True)
+        op = _op(labels, coeffs)
+        h = op._handle(L.QR_PHASE_ROWWISE)
+        p = orc.Params(labels, coeffs)
+        N = 1 << n
+        for r0 in [0, N // 3, N - 300]:
+            r1 = r0 + 300
+            ind = torch.empty(300 * h.G, dtype=torch.int64, device="cuda")
+            dat = torch.empty(300 * h.G, dtype=torch.complex128, device="cuda")
+            L.check(L.lib.qr_csr_build_rows(h.h, r0, r1, ctypes.c_void_p(ind.data_ptr()),
+                                            ctypes.c_void_p(dat.data_ptr()), None))
+            torch.cuda.synchronize()
+            _, ri, rd = orc.build_csr(p, row_begin=r0, row_end=r1, step=64)
+            assert ind.cpu().numpy().view(np.uint64).tobytes() == ri.tobytes()
+            assert dat.cpu().numpy().tobytes() == rd.tobytes()
+
+
+def test_single_term_and_identity_ops():
+    for labels in [["I"], ["Y"], ["Z" * 9], ["X" * 11], ["I" * 12, "I" * 12]]:
+        coeffs = [1.5 - 0.25j] * len(labels)
+        p = orc.Params(labels, coeffs)
+        ip, ind, dat = orc.build_csr(p)
+        _, data, indices, indptr = _export(_op(labels, coeffs).to_matrix_mode("Hip"))
+        assert indices.tobytes() == ind.tobytes() and data.tobytes() == dat.tobytes()
+
+
+# ---------------------------------------------------------------------------
+# the reference's Python tests for this path
+# ---------------------------------------------------------------------------
+I2 = np.array([[1.0, 0.0], [0.0, 1.0]], dtype=complex)
+Z2 = np.array([[1.0, 0.0], [0.0, -1.0]], dtype=complex)
+X2 = np.array([[0.0, 1.0], [1.0, 0.0]], dtype=complex)
+Y2 = np.array([[0.0, -1.0j], [1.0j, 0.0]], dtype=complex)
+
+
+def test_reference_test_H(hamiltonians):
+    """pyqrusty/tests/test_H.py:28-52."""
+    import qrusty_amd as qa
+    for name in ["py_H2", "py_H4", "py_H6"]:
+        H = _op(*hamiltonians[name])
+        m1 = qa.csr_matrix(H.to_matrix())
+        assert np.array_equal(m1.todense(), qa.csr_matrix(H.to_matrix_mode(mode="")).todense())
+        m2 = H.to_matrix_mode(mode="RowwiseUnsafeChunked/100")
+        assert np.array_equal(m1.todense(), qa.csr_matrix(m2).todense())
+    with pytest.raises(Exception):
+        _op(*hamiltonians["py_H2"]).to_matrix_mode("foo")
+    assert _op(*hamiltonians["py_H6"]).to_matrix().shape() == (1 << 12, 1 << 12)
+
+
+def test_reference_test_it_paulis_and_spmat():
+    """pyqrusty/tests/test_it.py:46-101, 218-230."""
+    import scipy.sparse as sps
+    import qrusty_amd as qa
+    for lab, M in [("I", I2), ("X", X2), ("Z", Z2), ("Y", Y2)]:
+        assert np.array_equal(M, qa.csr_matrix(qa.Pauli(lab).to_matrix()).todense())
+    spop = qa.SparsePauliOp([qa.Pauli("I"), qa.Pauli("Y")], [1.0 + 0.0j, 2.0 + 0.0j])
+    assert np.array_equal(I2 + 2.0 * Y2, qa.csr_matrix(spop.to_matrix()).todense())
+
+    dmat = np.array([[1.0, 2.0], [3.0, 4.0]], dtype=complex)
+    spmat = sps.csr_matrix(dmat)
+    spmat2 = qa.SpMat.new_unchecked(dmat.shape, spmat.data, spmat.indices, spmat.indptr)
+    assert repr(spmat2) == ("<2x2 sparse matrix of type Complex64\n\twith 4 stored elements in "
+                            "Compressed Sparse Row format>")
+    assert np.array_equal(spmat2.diagonal(), np.diagonal(dmat))
+    dmat3 = np.array([[1.0, 2.0], [3.0, 4.0], [5.0, 6.0]], dtype=complex)
+    s3 = sps.csr_matrix(dmat3)
+    spmat3 = qa.SpMat.new_unchecked(s3.shape, s3.data, s3.indices, s3.indptr)
+    assert repr(spmat3) == ("<3x2 sparse matrix of type Complex64\n\twith 6 stored elements in "
+                            "Compressed Sparse Row format>")
+    x2 = np.array([1.0 + 1j, -2.0 + 0.5j])
+    assert np.allclose(qa.spmat_dot_densevec(spmat3, x2), dmat3 @ x2, rtol=1e-15, atol=0)
+    qa.csr_matrix(spmat3)
+    with pytest.raises(Exception):
+        spmat3.export()
+    with pytest.raises(Exception):
+        qa.spmat_dot_densevec(spmat3, x2)
+    assert repr(spmat3) == "<already-dropped sparse matrix of type Complex64>"
+
+    pIX = qa.Pauli("IX")
+    matIX = pIX.to_matrix()
+    csr_matIX = qa.csr_matrix(pIX.to_matrix())
+    x = np.array([1.0 + 2j, 2.0 + 3j, 3.0 + 4j, 4.0 + 5j])
+    assert np.allclose(qa.spmat_dot_densevec(matIX, x), csr_matIX.dot(x))
+    with pytest.raises(Exception):
+        qa.spmat_dot_densevec(matIX, x[:3])
+
+
+def test_matvec_fixtures_vs_oracle(hamiltonians):
+    import scipy.sparse as sps
+    import qrusty_amd as qa
+    rng = np.random.default_rng(17)
+    for name in ["py_H6", "rs_H6", "py_H8"]:
+        labels, coeffs = hamiltonians[name]
+        print(f"[gpu] matvec {name}", flush=True)
+        m = _op(labels, coeffs).to_matrix_mode("Hip")
+        n = m.shape()[0]
+        x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        y = qa.spmat_dot_densevec(m, x)
+        shape, data, indices, indptr = _export(m)
+        ref = orc.spmat_dot_densevec(indptr, indices, data, x)
+        A = sps.csr_matrix((np.abs(data), indices.astype(np.int64), indptr.astype(np.int64)), shape=shape)
+        _check_matvec(y, ref, A.dot(np.abs(x)))
+
+
+def test_matvec_general_csr_shapes():
+    """Arbitrary CSR (empty rows, ragged rows, rectangular) through new_unchecked."""
+    import scipy.sparse as sps
+    import qrusty_amd as qa
+    rng = np.random.default_rng(5)
+    for rows, cols, dens in [(1, 1, 1.0), (7, 3, 0.5), (1000, 200, 0.01), (300, 5000, 0.2),
+                             (64, 64, 0.0)]:
+        A = sps.random(rows, cols, density=dens, format="csr", random_state=3,
+                       dtype=np.complex128,
+                       data_rvs=lambda k: rng.standard_normal(k) + 1j * rng.standard_normal(k))
+        A.sort_indices()
+        m = qa.SpMat.new_unchecked(A.shape, A.data, A.indices, A.indptr)
+        x = rng.standard_normal(cols) + 1j * rng.standard_normal(cols)
+        y = qa.spmat_dot_densevec(m, x)
+        ref = orc.spmat_dot_densevec(A.indptr.astype(np.uint64), A.indices.astype(np.uint64), A.data, x)
+        _check_matvec(y, ref, abs(A).dot(np.abs(x)))
+
+
+def test_device_tensor_matvec_stays_on_device(hamiltonians):
+    import qrusty_amd as qa
+    m = _op(*hamiltonians["py_H4"]).to_matrix_mode("Hip")
+    x = torch.randn(256, dtype=torch.complex128, device="cuda")
+    y = qa.spmat_dot_densevec(m, x)
+    assert y.is_cuda and y.shape == (256,)
+    ref = qa.spmat_dot_densevec(m, x.cpu().numpy())
+    assert np.array_equal(y.cpu().numpy(), ref)
