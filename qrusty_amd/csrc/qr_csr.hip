@@ -28,7 +28,7 @@ namespace qr {
 typedef double dvec2 __attribute__((ext_vector_type(2)));
 
 constexpr int kBuildThreads = 256;
-constexpr int kBuildItems = 8;  // entries per thread per block
+constexpr int kBuildM = 16;  // rows per thread per tile (4 varying row bits)
 
 template <bool kNarrow>
 __device__ __forceinline__ uint32_t parity_and(uint64_t a, uint64_t b) {
@@ -36,49 +36,81 @@ __device__ __forceinline__ uint32_t parity_and(uint64_t a, uint64_t b) {
     return __popcll(a & b) & 1u;
 }
 
+// (-1)^s * c, by flipping sign bits (exact, like the reference's -coeff)
+__device__ __forceinline__ double2 signed_coeff(double2 c, uint32_t s) {
+    if (s) { c.x = -c.x; c.y = -c.y; }
+    return c;
+}
+
+// Thread layout: a block has 256 threads = RS rows x JW groups (JW a power of
+// two >= min(G, 256)); blockIdx enumerates (row tile, group chunk).  A tile is
+// R = RS*16 aligned rows; thread (ro, lane) owns group j = chunk*JW + lane and
+// rows r = tile_base + ro + m*RS, m = 0..15.  Every bit of r except the four
+// m bits is fixed for the thread, so its rank list is walked once per tile:
+// entries on fixed bits are summed into rank_fixed, the (at most four) entries
+// on m bits go into a 16-entry offset table.  The group's first two terms are
+// held in registers; longer groups read the rest from the (cached) tables.
 template <bool kNarrow>
 __global__ __launch_bounds__(kBuildThreads) void k_csr_build(
-    uint64_t row_begin, uint64_t nrows, uint32_t G, uint32_t step_q, uint32_t step_r,
+    uint64_t row_begin, uint64_t row_end, uint32_t G, uint32_t jw_log2, uint32_t nchunks,
     const uint64_t *__restrict__ gx, const uint32_t *__restrict__ gterm,
     const uint64_t *__restrict__ tz, const double2 *__restrict__ tc,
     const uint32_t *__restrict__ rl_ptr, const RankEntry *__restrict__ rl,
     uint64_t *__restrict__ indices, double2 *__restrict__ data) {
-    const uint64_t total = nrows * uint64_t(G);
-    uint64_t item = uint64_t(blockIdx.x) * (kBuildThreads * kBuildItems) + threadIdx.x;
-    if (item >= total) return;
-    uint64_t lo = item / G;
-    uint32_t j = uint32_t(item - lo * G);
-#pragma unroll 1
-    for (int k = 0; k < kBuildItems; ++k) {
-        const uint64_t r = row_begin + lo;
-        // position of group j in row r
-        int64_t rank = int64_t(j);
-        const uint32_t e1 = rl_ptr[j + 1];
-        for (uint32_t e = rl_ptr[j]; e < e1; ++e) {
-            const RankEntry en = rl[e];
-            if ((r >> en.bit) & 1) rank += en.sigma;
-        }
-        // term-order fold of the group's signed coefficients
-        const uint32_t t0 = gterm[j], t1 = gterm[j + 1];
-        double2 c = tc[t0];
-        if (parity_and<kNarrow>(r, tz[t0])) { c.x = -c.x; c.y = -c.y; }
-        double2 v = c;
-        for (uint32_t t = t0 + 1; t < t1; ++t) {
-            c = tc[t];
-            if (parity_and<kNarrow>(r, tz[t])) { c.x = -c.x; c.y = -c.y; }
+    const uint32_t rs_log2 = 8u - jw_log2;
+    const uint32_t tile_log2 = rs_log2 + 4u;
+    const uint32_t chunk = blockIdx.x % nchunks;
+    const uint64_t tile = blockIdx.x / nchunks;
+    const uint32_t lane = threadIdx.x & ((1u << jw_log2) - 1u);
+    const uint32_t ro = threadIdx.x >> jw_log2;
+    const uint32_t j = (chunk << jw_log2) + lane;
+    if (j >= G) return;
+    const uint64_t rfix = (((row_begin >> tile_log2) + tile) << tile_log2) | ro;
+
+    int32_t rank_fixed = int32_t(j);
+    int32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    const uint32_t e1 = rl_ptr[j + 1];
+    for (uint32_t e = rl_ptr[j]; e < e1; ++e) {
+        const RankEntry en = rl[e];
+        const uint32_t b = uint32_t(en.bit);
+        const uint32_t q = b - rs_log2;  // wraps for b < rs_log2
+        if (q == 0) s0 = en.sigma;
+        else if (q == 1) s1 = en.sigma;
+        else if (q == 2) s2 = en.sigma;
+        else if (q == 3) s3 = en.sigma;
+        else if ((rfix >> b) & 1) rank_fixed += en.sigma;
+    }
+    const uint64_t xj = gx[j];
+    const uint32_t t0 = gterm[j], t1 = gterm[j + 1];
+    const uint64_t z0 = tz[t0];
+    const double2 c0 = tc[t0];
+    const bool multi = t1 > t0 + 1;
+    uint64_t z1 = 0;
+    double2 c1 = make_double2(0.0, 0.0);
+    if (multi) { z1 = tz[t0 + 1]; c1 = tc[t0 + 1]; }
+
+#pragma unroll
+    for (int m = 0; m < kBuildM; ++m) {
+        const uint64_t r = rfix + (uint64_t(m) << rs_log2);
+        if (r < row_begin || r >= row_end) continue;
+        const int32_t rank = rank_fixed + ((m & 1) ? s0 : 0) + ((m & 2) ? s1 : 0) +
+                             ((m & 4) ? s2 : 0) + ((m & 8) ? s3 : 0);
+        // term-order fold: first value, then + (accel.rs:191-208)
+        double2 v = signed_coeff(c0, parity_and<kNarrow>(r, z0));
+        if (multi) {
+            const double2 c = signed_coeff(c1, parity_and<kNarrow>(r, z1));
             v.x = v.x + c.x;
             v.y = v.y + c.y;
+            for (uint32_t t = t0 + 2; t < t1; ++t) {
+                const double2 ct = signed_coeff(tc[t], parity_and<kNarrow>(r, tz[t]));
+                v.x = v.x + ct.x;
+                v.y = v.y + ct.y;
+            }
         }
-        const uint64_t pos = lo * G + uint64_t(rank);
-        __builtin_nontemporal_store(r ^ gx[j], indices + pos);
-        dvec2 vv = {v.x, v.y};
+        const uint64_t pos = (r - row_begin) * G + uint32_t(rank);
+        __builtin_nontemporal_store(r ^ xj, indices + pos);
+        const dvec2 vv = {v.x, v.y};
         __builtin_nontemporal_store(vv, reinterpret_cast<dvec2 *>(data + pos));
-
-        item += kBuildThreads;
-        if (item >= total) break;
-        j += step_r;
-        lo += step_q;
-        if (j >= G) { j -= G; ++lo; }
     }
 }
 
@@ -91,23 +123,26 @@ __global__ __launch_bounds__(256) void k_indptr(uint64_t row_begin, uint64_t nro
 int launch_csr_build(const qr_op *op, const DeviceTables *t, uint64_t row_begin, uint64_t row_end,
                      uint64_t *d_indptr, uint64_t *d_indices, double *d_data, hipStream_t s) {
     const uint64_t nrows = row_end - row_begin;
+    if (op->G > 0x7FFFFFFFull) return fail(QR_ERR_SIZE, "too many groups");
     const uint32_t G = uint32_t(op->G);
-    if (op->G > 0xFFFFFFFFull) return fail(QR_ERR_SIZE, "too many groups");
     if (d_indptr) {
         uint64_t nb = (nrows + 1 + 255) / 256;
         hipLaunchKernelGGL(k_indptr, dim3(unsigned(nb)), dim3(256), 0, s, row_begin, nrows,
                            uint64_t(G), d_indptr);
         QR_HIP_CHECK(hipGetLastError());
     }
-    const uint64_t total = nrows * uint64_t(G);
-    const uint64_t per_block = uint64_t(kBuildThreads) * kBuildItems;
-    const uint64_t nblocks = (total + per_block - 1) / per_block;
+    if (nrows == 0) return QR_OK;
+    uint32_t jw_log2 = 0;
+    while ((1u << jw_log2) < G && jw_log2 < 8) ++jw_log2;
+    const uint32_t nchunks = (G + (1u << jw_log2) - 1) >> jw_log2;
+    const uint32_t tile_log2 = 8 - jw_log2 + 4;
+    const uint64_t ntiles = ((row_end - 1) >> tile_log2) - (row_begin >> tile_log2) + 1;
+    const uint64_t nblocks = ntiles * nchunks;
     if (nblocks > 0x7FFFFFFFull) return fail(QR_ERR_SIZE, "CSR build grid too large");
-    const uint32_t step_q = kBuildThreads / G, step_r = kBuildThreads % G;
     const bool narrow = op->nq <= 32;
     auto kern = narrow ? k_csr_build<true> : k_csr_build<false>;
-    hipLaunchKernelGGL(kern, dim3(unsigned(nblocks)), dim3(kBuildThreads), 0, s, row_begin, nrows,
-                       G, step_q, step_r, t->gx, t->gterm, t->tz, t->tc, t->rl_ptr, t->rl,
+    hipLaunchKernelGGL(kern, dim3(unsigned(nblocks)), dim3(kBuildThreads), 0, s, row_begin, row_end,
+                       G, jw_log2, nchunks, t->gx, t->gterm, t->tz, t->tc, t->rl_ptr, t->rl,
                        d_indices, reinterpret_cast<double2 *>(d_data));
     QR_HIP_CHECK(hipGetLastError());
     return QR_OK;
@@ -118,14 +153,24 @@ int launch_csr_build(const qr_op *op, const DeviceTables *t, uint64_t row_begin,
 // ---------------------------------------------------------------------------
 // Entries outside [0, nnz) and columns >= ncols are skipped, so a malformed
 // CSR handed in through SpMat.new_unchecked cannot read out of bounds.
+//
+// LPR lanes share a row, so a wave covers 64/LPR consecutive rows.  For the
+// Pauli CSR, rows that differ only in their low bits gather x at columns that
+// differ only in those bits, i.e. from the same cache lines, so a wave's
+// gathers coalesce.  Blocks are handed to the 8 XCDs round-robin; the block
+// index is remapped so that each XCD works on one contiguous range of rows
+// and its L2 sees the neighbouring rows' x lines.
 template <int LPR>
 __global__ __launch_bounds__(256) void k_csr_matvec(uint64_t nrows, uint64_t ncols, uint64_t nnz,
+                                                    uint32_t nblocks_xcd,
                                                     const uint64_t *__restrict__ indptr,
                                                     const uint64_t *__restrict__ indices,
                                                     const double2 *__restrict__ data,
                                                     const double2 *__restrict__ x,
                                                     double2 *__restrict__ y) {
-    const uint64_t gid = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    uint32_t b = blockIdx.x;
+    if (nblocks_xcd) b = (b % 8) * nblocks_xcd + b / 8;
+    const uint64_t gid = uint64_t(b) * 256 + threadIdx.x;
     const uint64_t row = gid / LPR;
     const int lane = int(threadIdx.x % LPR);
     double2 acc = make_double2(0.0, 0.0);
@@ -145,24 +190,23 @@ __global__ __launch_bounds__(256) void k_csr_matvec(uint64_t nrows, uint64_t nco
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                double2 b = make_double2(0.0, 0.0);
-                if (c[u] < ncols) b = x[c[u]];
-                acc.x = fma(a[u].x, b.x, acc.x);
-                acc.x = fma(-a[u].y, b.y, acc.x);
-                acc.y = fma(a[u].x, b.y, acc.y);
-                acc.y = fma(a[u].y, b.x, acc.y);
+                double2 bx = make_double2(0.0, 0.0);
+                if (c[u] < ncols) bx = x[c[u]];
+                acc.x = fma(a[u].x, bx.x, acc.x);
+                acc.x = fma(-a[u].y, bx.y, acc.x);
+                acc.y = fma(a[u].x, bx.y, acc.y);
+                acc.y = fma(a[u].y, bx.x, acc.y);
             }
         }
         for (; k < k1; k += LPR) {
             const uint64_t c = __builtin_nontemporal_load(indices + k);
             const dvec2 av = __builtin_nontemporal_load(reinterpret_cast<const dvec2 *>(data + k));
-            const double2 a = make_double2(av.x, av.y);
-            double2 b = make_double2(0.0, 0.0);
-            if (c < ncols) b = x[c];
-            acc.x = fma(a.x, b.x, acc.x);
-            acc.x = fma(-a.y, b.y, acc.x);
-            acc.y = fma(a.x, b.y, acc.y);
-            acc.y = fma(a.y, b.x, acc.y);
+            double2 bx = make_double2(0.0, 0.0);
+            if (c < ncols) bx = x[c];
+            acc.x = fma(av.x, bx.x, acc.x);
+            acc.x = fma(-av.y, bx.y, acc.x);
+            acc.y = fma(av.x, bx.y, acc.y);
+            acc.y = fma(av.y, bx.x, acc.y);
         }
     }
 #pragma unroll
@@ -177,24 +221,29 @@ int launch_csr_matvec(uint64_t nrows, uint64_t ncols, uint64_t nnz, const uint64
                       const uint64_t *d_indices, const double *d_data, const double *d_x,
                       double *d_y, hipStream_t s) {
     const uint64_t avg = nrows ? nnz / nrows : 0;
+    // ~4+ entries per lane, at most 8 lanes per row (8 rows per wave)
     int lpr = 1;
-    while (lpr < 64 && uint64_t(lpr) * 2 <= avg) lpr *= 2;  // ~2+ entries per lane
+    while (lpr < 8 && uint64_t(lpr) * 4 <= avg) lpr *= 2;
     const uint64_t threads = nrows * uint64_t(lpr);
     const uint64_t nblocks = (threads + 255) / 256;
     if (nblocks > 0x7FFFFFFFull) return fail(QR_ERR_SIZE, "CSR matvec grid too large");
+    const uint32_t nbx = (nblocks % 8 == 0) ? uint32_t(nblocks / 8) : 0;
     auto x = reinterpret_cast<const double2 *>(d_x);
     auto data = reinterpret_cast<const double2 *>(d_data);
     auto y = reinterpret_cast<double2 *>(d_y);
     const dim3 grid(static_cast<unsigned>(nblocks)), block(256);
+#define QR_MV_CASE(L)                                                                        \
+    case L:                                                                                  \
+        hipLaunchKernelGGL(k_csr_matvec<L>, grid, block, 0, s, nrows, ncols, nnz, nbx,       \
+                           d_indptr, d_indices, data, x, y);                                 \
+        break;
     switch (lpr) {
-        case 1: hipLaunchKernelGGL(k_csr_matvec<1>, grid, block, 0, s, nrows, ncols, nnz, d_indptr, d_indices, data, x, y); break;
-        case 2: hipLaunchKernelGGL(k_csr_matvec<2>, grid, block, 0, s, nrows, ncols, nnz, d_indptr, d_indices, data, x, y); break;
-        case 4: hipLaunchKernelGGL(k_csr_matvec<4>, grid, block, 0, s, nrows, ncols, nnz, d_indptr, d_indices, data, x, y); break;
-        case 8: hipLaunchKernelGGL(k_csr_matvec<8>, grid, block, 0, s, nrows, ncols, nnz, d_indptr, d_indices, data, x, y); break;
-        case 16: hipLaunchKernelGGL(k_csr_matvec<16>, grid, block, 0, s, nrows, ncols, nnz, d_indptr, d_indices, data, x, y); break;
-        case 32: hipLaunchKernelGGL(k_csr_matvec<32>, grid, block, 0, s, nrows, ncols, nnz, d_indptr, d_indices, data, x, y); break;
-        default: hipLaunchKernelGGL(k_csr_matvec<64>, grid, block, 0, s, nrows, ncols, nnz, d_indptr, d_indices, data, x, y); break;
+        QR_MV_CASE(1)
+        QR_MV_CASE(2)
+        QR_MV_CASE(4)
+        default: QR_MV_CASE(8)
     }
+#undef QR_MV_CASE
     QR_HIP_CHECK(hipGetLastError());
     return QR_OK;
 }
