@@ -102,7 +102,46 @@ static void free_tables(DeviceTables &t) {
     void *ptrs[] = {t.gx, t.gterm, t.tz, t.tc, t.rl_ptr, t.rl, t.gkind, t.partials};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    for (auto &pt : t.pat) {
+        void *pp[] = {pt.gpat, pt.pzl, pt.ptptr, pt.ptz, pt.ptc};
+        for (void *p : pp)
+            if (p) (void)hipFree(p);
+    }
     t = DeviceTables{};
+}
+
+// Patterns: inside a tile of 2^L rows (fixed high bits h, low bits l), term t
+// of group g contributes (-1)^popcount(h & z_t) * c'_t * (-1)^popcount(l & zl_t)
+// with zl_t = z_t & (2^L - 1).  Terms of a group that share zl_t therefore
+// merge, per tile, into one coefficient.  Patterns keep first-occurrence order
+// and their terms keep input order.
+void build_patterns(const qr_op *op, int L, HostPatterns &out) {
+    const uint64_t mask = L >= 64 ? ~uint64_t(0) : ((uint64_t(1) << L) - 1);
+    out = HostPatterns{};
+    out.gpat.push_back(0);
+    out.ptptr.push_back(0);
+    std::vector<uint32_t> order;
+    for (uint64_t g = 0; g < op->G; ++g) {
+        std::vector<uint64_t> zls;
+        std::vector<std::vector<uint32_t>> members;
+        for (uint32_t i = op->gterm[g]; i < op->gterm[g + 1]; ++i) {
+            const uint64_t zl = op->tz[i] & mask;
+            size_t k = 0;
+            while (k < zls.size() && zls[k] != zl) ++k;
+            if (k == zls.size()) { zls.push_back(zl); members.emplace_back(); }
+            members[k].push_back(i);
+        }
+        for (size_t k = 0; k < zls.size(); ++k) {
+            out.pzl.push_back(uint32_t(zls[k]));
+            for (uint32_t i : members[k]) {
+                out.ptz.push_back(op->tz[i]);
+                out.ptc.push_back(op->tc[2 * i]);
+                out.ptc.push_back(op->tc[2 * i + 1]);
+            }
+            out.ptptr.push_back(uint32_t(out.ptz.size()));
+        }
+        out.gpat.push_back(uint32_t(out.pzl.size()));
+    }
 }
 
 template <typename T>
@@ -134,6 +173,27 @@ int device_tables(const qr_op *op, DeviceTables **out) {
         t.ready = true;
     }
     *out = &t;
+    return QR_OK;
+}
+
+int device_patterns(const qr_op *op, DeviceTables *t, int slot, int L) {
+    std::lock_guard<std::mutex> lock(op->mu);
+    DeviceTables::Patterns &pt = t->pat[slot];
+    if (pt.L == L) return QR_OK;
+    for (void *p : {(void *)pt.gpat, (void *)pt.pzl, (void *)pt.ptptr, (void *)pt.ptz, (void *)pt.ptc})
+        if (p) (void)hipFree(p);
+    pt = DeviceTables::Patterns{};
+    HostPatterns hp;
+    build_patterns(op, L, hp);
+    std::vector<double2> c2(hp.ptz.size());
+    for (size_t i = 0; i < c2.size(); ++i) c2[i] = make_double2(hp.ptc[2 * i], hp.ptc[2 * i + 1]);
+    int rc;
+    if ((rc = upload(&pt.gpat, hp.gpat)) || (rc = upload(&pt.pzl, hp.pzl)) ||
+        (rc = upload(&pt.ptptr, hp.ptptr)) || (rc = upload(&pt.ptz, hp.ptz)) ||
+        (rc = upload(&pt.ptc, c2)))
+        return rc;
+    pt.P = uint32_t(hp.pzl.size());
+    pt.L = L;
     return QR_OK;
 }
 

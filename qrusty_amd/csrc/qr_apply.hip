@@ -4,27 +4,57 @@
 //
 // This equals the rowwise CSR build followed by the CSR matvec
 // (qrusty/src/accel.rs:171-210, 338-370) without materialising the 2^n*G
-// entries: D_g(r) is the CSR value of group g in row r (same term-order fold),
-// and the gather x[r ^ x_g] is the CSR column.  One thread owns one row; the
-// group/term tables are read with wave-uniform (scalar) loads; D_g(r) is
-// computed once per row and reused for every state vector of the launch.
+// entries: D_g(r) is the CSR value of group g in row r, x[r ^ x_g] the gather
+// of its column.
+//
+// Work decomposition (one block = one tile of 2^L consecutive rows of one
+// state vector):
+//  * The tile's slice of x is staged in LDS (coalesced); groups whose x mask
+//    is below 2^L gather from LDS, the others ("far" groups) gather the
+//    partner tile t ^ (x_g >> L) from global memory (coalesced, permuted
+//    inside a line).
+//  * Blocks are ordered vector-major, and each XCD receives one contiguous
+//    range of tiles of the vector: the far partners of nearby tiles are read
+//    through the same L2, and one complex64 vector (128 MiB at 24 qubits)
+//    stays resident in the 256 MB MALL while the chip works on it.
+//  * Row r = h + l (h = tile base, l < 2^L):  (-1)^popcount(r & z) =
+//    (-1)^popcount(h & z) * (-1)^popcount(l & zl).  The host merges the terms
+//    of a group that share zl into "patterns"; per tile each pattern's
+//    coefficient A_p = sum (-1)^popcount(h & z_t) c'_t is formed once (wave-
+//    uniform), and per row only patterns with zl != 0 cost a sign.  The block
+//    forms all its patterns' coefficients in parallel at the start of the
+//    tile and keeps them in LDS next to the x tile.  Thread `tid` owns rows
+//    l = tid + 1024*k: the sign splits again into a per-thread part (bits of
+//    tid) and a per-k part (wave-uniform), so each pattern costs one FMA per
+//    row.
 //
 // Precision modes
-//   FP64 : complex128 state, FP64 fused multiply-adds.
-//   MIXED: complex64 state.  D_g(r) is merged in FP64 (the cancellation between
-//          terms sharing an x mask happens there) and rounded to FP32; the
-//          product with x and the sum over groups run in FP32, the sum with
-//          TwoSum compensation.  <x|y> is accumulated in FP64 from the
-//          uncompensated-sum pair (s + c, exact in FP64).
-// <x|y> is reduced deterministically: per-thread, then per block (fixed tree),
-// then one final pass over the block partials.
+//   FP64 : complex128 state, FP64 FMAs (tile 2^12 rows, 4 rows per thread).
+//   MIXED: complex64 state (tile 2^13 rows, 8 rows per thread).  D_g(r) is
+//          merged in FP64 -- where terms sharing an x mask cancel -- and then
+//          rounded to FP32.  Products and the sum over groups run in FP32:
+//          products are accumulated into a block partial over up to
+//          kCompBlock groups, and block partials are added to the running sum
+//          with TwoSum (s + e == a + b exactly), the error terms collected in a
+//          separate compensation.  <x|y> is reduced in FP64 from s + c.
+// <x|y> is reduced deterministically: per thread, per block (fixed tree),
+// then one pass over the block partials of each vector.
 #include "qr_internal.h"
 
 #include <type_traits>
 
 namespace qr {
 
-constexpr int kApplyThreads = 256;
+constexpr int kApplyThreads = 1024;
+constexpr int kApplyThreadBits = 10;
+constexpr int kCompBlock = 8;       // groups per FP32 partial between TwoSum steps
+constexpr uint32_t kMaxPatLDS = 4096;  // per-tile pattern coefficients kept in LDS
+
+template <bool kMixed>
+struct ApplyShape {
+    static constexpr int L = kMixed ? 13 : 12;           // tile = 2^L rows
+    static constexpr int R = (1 << L) / kApplyThreads;  // rows per thread
+};
 
 template <bool kNarrow>
 __device__ __forceinline__ uint32_t parity_of(uint64_t a, uint64_t b) {
@@ -46,146 +76,218 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Reduces NV complex values over the block; thread 0 writes them to out[2*v].
-template <int NV>
-__device__ __forceinline__ void block_reduce_store(double (&er)[NV], double (&ei)[NV],
-                                                   double *out) {
-    __shared__ double red[kApplyThreads / 64][NV][2];
-    const int lane = threadIdx.x % 64, wid = threadIdx.x / 64;
+// One group for the R rows of a thread: D_g(r) from the tile's pattern
+// coefficients, then the gather of x[r ^ x_g] and the product into the
+// accumulators (FP64: ar/ai; mixed: the FP32 block partial br/bi).
+template <bool kMixed, bool kReal, int R, typename cT>
+__device__ __forceinline__ void apply_group(
+    uint32_t g, uint32_t tid, uint64_t tile_rows, uint64_t base, const uint64_t *__restrict__ gx,
+    const uint32_t *__restrict__ gpat, const uint32_t *__restrict__ pzl,
+    const uint32_t *__restrict__ ptptr, const uint64_t *__restrict__ ptz,
+    const double2 *__restrict__ ptc, bool pat_lds, const double2 *sA, const cT *xt,
+    const cT *__restrict__ xv, double (&ar)[R], double (&ai)[R], float (&br)[R],
+    float (&bi)[R]) {
+    const uint64_t m = gx[g];
+    double dr[R], di[R];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-        const double a = wave_sum(er[v]), b = wave_sum(ei[v]);
-        if (lane == 0) { red[wid][v][0] = a; red[wid][v][1] = b; }
+    for (int k = 0; k < R; ++k) dr[k] = di[k] = 0.0;
+    const uint32_t p1 = gpat[g + 1];
+    for (uint32_t p = gpat[g]; p < p1; ++p) {
+        // tile coefficient of the pattern (wave-uniform)
+        double a_r = 0.0, a_i = 0.0;
+        if (pat_lds) {
+            const double2 a = sA[p];
+            a_r = a.x;
+            a_i = a.y;
+        } else {
+            const uint32_t t1 = ptptr[p + 1];
+            for (uint32_t t = ptptr[p]; t < t1; ++t) {
+                const double2 c = ptc[t];
+                if (__popcll(base & ptz[t]) & 1) { a_r -= c.x; a_i -= c.y; }
+                else { a_r += c.x; a_i += c.y; }
+            }
+        }
+        const uint32_t zl = pzl[p];
+        if (zl == 0) {
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                dr[k] += a_r;
+                if (!kReal) di[k] += a_i;
+            }
+        } else {
+            const uint32_t st = __popc(tid & zl) & 1u;  // per-thread sign
+            const double vr = st ? -a_r : a_r, vi = st ? -a_i : a_i;
+            const uint32_t zk = zl >> kApplyThreadBits;
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                const double sg = (__popc(uint32_t(k) & zk) & 1u) ? -1.0 : 1.0;  // uniform
+                dr[k] = fma(sg, vr, dr[k]);
+                if (!kReal) di[k] = fma(sg, vi, di[k]);
+            }
+        }
+    }
+    // gather partner rows: l ^ (m & (2^L-1)) inside tile base ^ (m >> L)
+    const uint64_t mlo = m & (tile_rows - 1), mhi = m & ~(tile_rows - 1);
+    const uint32_t tx = tid ^ uint32_t(mlo & (kApplyThreads - 1));
+    const uint32_t mk = uint32_t(mlo >> kApplyThreadBits);
+    const cT *__restrict__ far = xv + (base ^ mhi);
+    cT xg[R];
+    if (mhi) {
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const uint64_t l = tid + uint64_t(k) * kApplyThreads;
+            if (l < tile_rows) xg[k] = far[tx + ((uint32_t(k) ^ mk) << kApplyThreadBits)];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const uint64_t l = tid + uint64_t(k) * kApplyThreads;
+            if (l < tile_rows) xg[k] = xt[tx + ((uint32_t(k) ^ mk) << kApplyThreadBits)];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const uint64_t l = tid + uint64_t(k) * kApplyThreads;
+        if (l >= tile_rows) break;
+        const cT xb = xg[k];
+        if (!kMixed) {
+            ar[k] = fma(dr[k], xb.x, ar[k]);
+            ai[k] = fma(dr[k], xb.y, ai[k]);
+            if (!kReal) {
+                ar[k] = fma(-di[k], xb.y, ar[k]);
+                ai[k] = fma(di[k], xb.x, ai[k]);
+            }
+        } else {
+            const float fr = float(dr[k]);
+            br[k] = fmaf(fr, xb.x, br[k]);
+            bi[k] = fmaf(fr, xb.y, bi[k]);
+            if (!kReal) {
+                const float fi = float(di[k]);
+                br[k] = fmaf(-fi, xb.y, br[k]);
+                bi[k] = fmaf(fi, xb.x, bi[k]);
+            }
+        }
+    }
+}
+
+template <bool kMixed, bool kNarrow>
+__global__ __launch_bounds__(kApplyThreads) void k_apply_tiled(
+    uint64_t N, uint32_t G, uint32_t P, uint32_t ntiles, uint32_t tiles_per_xcd,
+    const uint64_t *__restrict__ gx, const uint8_t *__restrict__ gkind,
+    const uint32_t *__restrict__ gpat, const uint32_t *__restrict__ pzl,
+    const uint32_t *__restrict__ ptptr, const uint64_t *__restrict__ ptz,
+    const double2 *__restrict__ ptc, const void *__restrict__ xin, void *__restrict__ yout,
+    double *__restrict__ partials) {
+    using Shape = ApplyShape<kMixed>;
+    constexpr int L = Shape::L, R = Shape::R;
+    using cT = typename std::conditional<kMixed, float2, double2>::type;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cT *xt = reinterpret_cast<cT *>(smem_raw);
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t v = blockIdx.x / ntiles;
+    uint32_t tile = blockIdx.x % ntiles;
+    if (tiles_per_xcd) tile = (tile % 8) * tiles_per_xcd + tile / 8;
+    const uint64_t tile_rows = N < (uint64_t(1) << L) ? N : (uint64_t(1) << L);
+    const uint64_t base = uint64_t(tile) * tile_rows;
+    const cT *__restrict__ xv = reinterpret_cast<const cT *>(xin) + uint64_t(v) * N;
+
+    for (uint64_t i = tid; i < tile_rows; i += kApplyThreads) xt[i] = xv[base + i];
+    // per-tile pattern coefficients A_p = sum_t (-1)^popcount(base & z_t) c'_t
+    double2 *sA = reinterpret_cast<double2 *>(smem_raw + tile_rows * sizeof(cT));
+    const bool pat_lds = P <= kMaxPatLDS;
+    if (pat_lds) {
+        for (uint32_t p = tid; p < P; p += kApplyThreads) {
+            double a_r = 0.0, a_i = 0.0;
+            const uint32_t t1 = ptptr[p + 1];
+            for (uint32_t t = ptptr[p]; t < t1; ++t) {
+                const double2 c = ptc[t];
+                if (parity_of<kNarrow>(base, ptz[t])) { a_r -= c.x; a_i -= c.y; }
+                else { a_r += c.x; a_i += c.y; }
+            }
+            sA[p] = make_double2(a_r, a_i);
+        }
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+
+    // accumulators: FP64 (ar, ai) or FP32 running sum s, compensation c, block partial b
+    double ar[R], ai[R];
+    float sr[R], si[R], cr[R], ci[R], br[R], bi[R];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            double a = 0.0, b = 0.0;
-            for (int w = 0; w < kApplyThreads / 64; ++w) { a += red[w][v][0]; b += red[w][v][1]; }
-            out[2 * v] = a;
-            out[2 * v + 1] = b;
+    for (int k = 0; k < R; ++k) {
+        ar[k] = ai[k] = 0.0;
+        sr[k] = si[k] = cr[k] = ci[k] = br[k] = bi[k] = 0.0f;
+    }
+
+    int since_flush = 0;
+    for (uint32_t g = 0; g < G; ++g) {
+        // kind 1: every c' of the group is real, so D_g(r) is real
+        if (gkind[g] == 1)
+            apply_group<kMixed, true, R>(g, tid, tile_rows, base, gx, gpat, pzl, ptptr, ptz, ptc,
+                                         pat_lds, sA, xt, xv, ar, ai, br, bi);
+        else
+            apply_group<kMixed, false, R>(g, tid, tile_rows, base, gx, gpat, pzl, ptptr, ptz, ptc,
+                                          pat_lds, sA, xt, xv, ar, ai, br, bi);
+        if (kMixed && (++since_flush == kCompBlock || g + 1 == G)) {
+            since_flush = 0;
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                float e;
+                two_sum(sr[k], br[k], sr[k], e);
+                cr[k] += e;
+                two_sum(si[k], bi[k], si[k], e);
+                ci[k] += e;
+                br[k] = bi[k] = 0.0f;
+            }
+        }
+    }
+
+    double er = 0.0, ei = 0.0;
+    cT *__restrict__ yv = yout ? reinterpret_cast<cT *>(yout) + uint64_t(v) * N : nullptr;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const uint64_t l = tid + uint64_t(k) * kApplyThreads;
+        if (l >= tile_rows) break;
+        double yr, yi;
+        if (!kMixed) {
+            yr = ar[k];
+            yi = ai[k];
+            if (yv) reinterpret_cast<double2 *>(yv)[base + l] = make_double2(yr, yi);
+        } else {
+            yr = double(sr[k]) + double(cr[k]);
+            yi = double(si[k]) + double(ci[k]);
+            if (yv) reinterpret_cast<float2 *>(yv)[base + l] = make_float2(sr[k] + cr[k], si[k] + ci[k]);
+        }
+        if (partials) {
+            const cT xr = xt[l];
+            const double xa = xr.x, xb = xr.y;
+            er = fma(xa, yr, er); er = fma(xb, yi, er);
+            ei = fma(xa, yi, ei); ei = fma(-xb, yr, ei);
+        }
+    }
+    if (partials) {
+        __shared__ double red[kApplyThreads / 64][2];
+        const double a = wave_sum(er), b = wave_sum(ei);
+        if ((tid & 63) == 0) { red[tid / 64][0] = a; red[tid / 64][1] = b; }
+        __syncthreads();
+        if (tid == 0) {
+            double s0 = 0.0, s1 = 0.0;
+            for (int w = 0; w < kApplyThreads / 64; ++w) { s0 += red[w][0]; s1 += red[w][1]; }
+            partials[2 * uint64_t(blockIdx.x)] = s0;
+            partials[2 * uint64_t(blockIdx.x) + 1] = s1;
         }
     }
 }
 
-template <int NV, bool kMixed, bool kNarrow>
-__global__ __launch_bounds__(kApplyThreads) void k_apply(
-    uint64_t N, uint32_t G, uint64_t rows_per_block, uint32_t nblocks_xcd,
-    const uint64_t *__restrict__ gx, const uint32_t *__restrict__ gterm,
-    const uint64_t *__restrict__ tz, const double2 *__restrict__ tc,
-    const uint8_t *__restrict__ gkind, const void *__restrict__ xin, void *__restrict__ yout,
-    double *__restrict__ partials, uint32_t nvec_total, uint32_t v0) {
-    using cT = typename std::conditional<kMixed, float2, double2>::type;
-    const cT *__restrict__ x = reinterpret_cast<const cT *>(xin) + uint64_t(v0) * N;
-    cT *__restrict__ y = yout ? reinterpret_cast<cT *>(yout) + uint64_t(v0) * N : nullptr;
-
-    // blocks are dealt round-robin to the 8 XCDs; give each XCD a contiguous
-    // range of rows so partner rows r ^ x_g of nearby rows share its L2
-    uint32_t b = blockIdx.x;
-    if (nblocks_xcd) b = (b % 8) * nblocks_xcd + b / 8;
-    const uint64_t row0 = uint64_t(b) * rows_per_block;
-    const uint64_t row1 = row0 + rows_per_block < N ? row0 + rows_per_block : N;
-
-    double er[NV], ei[NV];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) { er[v] = 0.0; ei[v] = 0.0; }
-
-    for (uint64_t r = row0 + threadIdx.x; r < row1; r += kApplyThreads) {
-        double ar[NV], ai[NV];
-        float sr[NV], si[NV], cr[NV], ci[NV];
-#pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            ar[v] = ai[v] = 0.0;
-            sr[v] = si[v] = cr[v] = ci[v] = 0.0f;
-        }
-        for (uint32_t g = 0; g < G; ++g) {
-            const uint32_t t0 = gterm[g], t1 = gterm[g + 1];
-            double2 c = tc[t0];
-            if (parity_of<kNarrow>(r, tz[t0])) { c.x = -c.x; c.y = -c.y; }
-            double dr = c.x, di = c.y;
-            for (uint32_t t = t0 + 1; t < t1; ++t) {
-                c = tc[t];
-                if (parity_of<kNarrow>(r, tz[t])) { c.x = -c.x; c.y = -c.y; }
-                dr = dr + c.x;
-                di = di + c.y;
-            }
-            const uint64_t col = r ^ gx[g];
-            const uint8_t kind = gkind[g];
-            if (!kMixed) {
-#pragma unroll
-                for (int v = 0; v < NV; ++v) {
-                    const double2 b2 = reinterpret_cast<const double2 *>(x)[uint64_t(v) * N + col];
-                    if (kind == 1) {
-                        ar[v] = fma(dr, b2.x, ar[v]);
-                        ai[v] = fma(dr, b2.y, ai[v]);
-                    } else {
-                        ar[v] = fma(dr, b2.x, ar[v]);
-                        ar[v] = fma(-di, b2.y, ar[v]);
-                        ai[v] = fma(dr, b2.y, ai[v]);
-                        ai[v] = fma(di, b2.x, ai[v]);
-                    }
-                }
-            } else {
-                const float fr = float(dr), fi = float(di);
-#pragma unroll
-                for (int v = 0; v < NV; ++v) {
-                    const float2 b2 = reinterpret_cast<const float2 *>(x)[uint64_t(v) * N + col];
-                    float pr, pi;
-                    if (kind == 1) {
-                        pr = fr * b2.x;
-                        pi = fr * b2.y;
-                    } else {
-                        pr = fmaf(fr, b2.x, -fi * b2.y);
-                        pi = fmaf(fr, b2.y, fi * b2.x);
-                    }
-                    float e;
-                    two_sum(sr[v], pr, sr[v], e);
-                    cr[v] += e;
-                    two_sum(si[v], pi, si[v], e);
-                    ci[v] += e;
-                }
-            }
-        }
-#pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            double yr, yi;
-            if (!kMixed) {
-                yr = ar[v];
-                yi = ai[v];
-                if (y) reinterpret_cast<double2 *>(y)[uint64_t(v) * N + r] = make_double2(yr, yi);
-                const double2 xr = reinterpret_cast<const double2 *>(x)[uint64_t(v) * N + r];
-                if (partials) {
-                    er[v] = fma(xr.x, yr, er[v]); er[v] = fma(xr.y, yi, er[v]);
-                    ei[v] = fma(xr.x, yi, ei[v]); ei[v] = fma(-xr.y, yr, ei[v]);
-                }
-            } else {
-                yr = double(sr[v]) + double(cr[v]);
-                yi = double(si[v]) + double(ci[v]);
-                if (y) reinterpret_cast<float2 *>(y)[uint64_t(v) * N + r] =
-                           make_float2(sr[v] + cr[v], si[v] + ci[v]);
-                const float2 xr = reinterpret_cast<const float2 *>(x)[uint64_t(v) * N + r];
-                if (partials) {
-                    const double xa = xr.x, xb = xr.y;
-                    er[v] = fma(xa, yr, er[v]); er[v] = fma(xb, yi, er[v]);
-                    ei[v] = fma(xa, yi, ei[v]); ei[v] = fma(-xb, yr, ei[v]);
-                }
-            }
-        }
-    }
-    if (partials)
-        block_reduce_store<NV>(er, ei, partials + (uint64_t(blockIdx.x) * nvec_total + v0) * 2);
-}
-
-// out[2v], out[2v+1] = sum over blocks of partials[(blk*nvec + v)*2 + {0,1}]
-__global__ __launch_bounds__(256) void k_reduce_partials(uint32_t nblocks, uint32_t nvec,
+// out[2v], out[2v+1] = sum of the ntiles block partials of vector v
+__global__ __launch_bounds__(256) void k_reduce_partials(uint32_t ntiles,
                                                          const double *__restrict__ partials,
                                                          double *__restrict__ out) {
     const uint32_t v = blockIdx.x;
     double a = 0.0, b = 0.0;
-    for (uint32_t i = threadIdx.x; i < nblocks; i += 256) {
-        a += partials[(uint64_t(i) * nvec + v) * 2];
-        b += partials[(uint64_t(i) * nvec + v) * 2 + 1];
+    for (uint32_t i = threadIdx.x; i < ntiles; i += 256) {
+        a += partials[2 * (uint64_t(v) * ntiles + i)];
+        b += partials[2 * (uint64_t(v) * ntiles + i) + 1];
     }
     __shared__ double ra[256], rb[256];
     ra[threadIdx.x] = a;
@@ -202,36 +304,40 @@ __global__ __launch_bounds__(256) void k_reduce_partials(uint32_t nblocks, uint3
 }
 
 template <bool kMixed, bool kNarrow>
-static void launch_chunk(int nv, dim3 grid, hipStream_t s, uint64_t N, uint32_t G,
-                         uint64_t rpb, uint32_t nbx, const DeviceTables *t, const void *x, void *y,
-                         double *partials, uint32_t nvec, uint32_t v0) {
-#define QR_APPLY_CASE(NVV)                                                                  \
-    case NVV:                                                                               \
-        hipLaunchKernelGGL((k_apply<NVV, kMixed, kNarrow>), grid, dim3(kApplyThreads), 0, s, \
-                           N, G, rpb, nbx, t->gx, t->gterm, t->tz, t->tc, t->gkind, x, y,   \
-                           partials, nvec, v0);                                             \
-        break;
-    switch (nv) {
-        QR_APPLY_CASE(1)
-        QR_APPLY_CASE(2)
-        QR_APPLY_CASE(4)
-        default: QR_APPLY_CASE(8)
-    }
-#undef QR_APPLY_CASE
+static int launch_tiled(const qr_op *op, DeviceTables *t, uint32_t nvec, const void *d_x,
+                        void *d_y, double *partials, uint32_t ntiles, hipStream_t s) {
+    using Shape = ApplyShape<kMixed>;
+    using cT = typename std::conditional<kMixed, float2, double2>::type;
+    const uint64_t N = uint64_t(1) << op->nq;
+    const uint64_t tile_rows = N < (uint64_t(1) << Shape::L) ? N : (uint64_t(1) << Shape::L);
+    const DeviceTables::Patterns &pt = t->pat[kMixed ? 1 : 0];
+    const size_t shmem = size_t(tile_rows) * sizeof(cT) +
+                         (pt.P <= kMaxPatLDS ? size_t(pt.P) * sizeof(double2) : 0);
+    const uint32_t tpx = (ntiles % 8 == 0) ? ntiles / 8 : 0;
+    auto kern = k_apply_tiled<kMixed, kNarrow>;
+    QR_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem)));
+    const uint64_t nblocks = uint64_t(ntiles) * nvec;
+    if (nblocks > 0x7FFFFFFFull) return fail(QR_ERR_SIZE, "apply grid too large");
+    hipLaunchKernelGGL(kern, dim3(unsigned(nblocks)), dim3(kApplyThreads), shmem, s, N,
+                       uint32_t(op->G), pt.P, ntiles, tpx, t->gx, t->gkind, pt.gpat, pt.pzl, pt.ptptr,
+                       pt.ptz, pt.ptc, d_x, d_y, partials);
+    QR_HIP_CHECK(hipGetLastError());
+    return QR_OK;
 }
 
 int launch_pauli_apply(const qr_op *op, DeviceTables *t, int32_t precision, uint32_t nvec,
                        const void *d_x, void *d_y, double *d_expect, hipStream_t s) {
+    const bool mixed = precision == QR_PREC_MIXED, narrow = op->nq <= 32;
+    const int Lmax = mixed ? ApplyShape<true>::L : ApplyShape<false>::L;
+    const int L = int(op->nq) < Lmax ? int(op->nq) : Lmax;
+    int rc = device_patterns(op, t, mixed ? 1 : 0, L);
+    if (rc) return rc;
     const uint64_t N = uint64_t(1) << op->nq;
-    const uint32_t G = uint32_t(op->G);
-    // about 2048 blocks (8 per CU); rows per block a multiple of the block size
-    uint64_t rpb = (N + 2047) / 2048;
-    rpb = (rpb + kApplyThreads - 1) / kApplyThreads * kApplyThreads;
-    const uint64_t nblocks = (N + rpb - 1) / rpb;
-    const uint32_t nbx = (nblocks % 8 == 0) ? uint32_t(nblocks / 8) : 0;
+    const uint32_t ntiles = uint32_t(N >> L);
     double *partials = nullptr;
     if (d_expect) {
-        const size_t need = size_t(nblocks) * nvec * 2;
+        const size_t need = size_t(ntiles) * nvec * 2;
         if (t->partials_cap < need) {
             if (t->partials) (void)hipFree(t->partials);
             t->partials = nullptr;
@@ -241,26 +347,14 @@ int launch_pauli_apply(const qr_op *op, DeviceTables *t, int32_t precision, uint
         }
         partials = t->partials;
     }
-    const bool mixed = precision == QR_PREC_MIXED, narrow = op->nq <= 32;
-    for (uint32_t v0 = 0; v0 < nvec;) {
-        const uint32_t left = nvec - v0;
-        const int nv = left >= 8 ? 8 : left >= 4 ? 4 : left >= 2 ? 2 : 1;
-        const void *x = d_x;
-        void *y = d_y;
-        const dim3 grid(static_cast<unsigned>(nblocks));
-        if (mixed) {
-            if (narrow) launch_chunk<true, true>(nv, grid, s, N, G, rpb, nbx, t, x, y, partials, nvec, v0);
-            else launch_chunk<true, false>(nv, grid, s, N, G, rpb, nbx, t, x, y, partials, nvec, v0);
-        } else {
-            if (narrow) launch_chunk<false, true>(nv, grid, s, N, G, rpb, nbx, t, x, y, partials, nvec, v0);
-            else launch_chunk<false, false>(nv, grid, s, N, G, rpb, nbx, t, x, y, partials, nvec, v0);
-        }
-        QR_HIP_CHECK(hipGetLastError());
-        v0 += uint32_t(nv);
-    }
+    if (mixed) rc = narrow ? launch_tiled<true, true>(op, t, nvec, d_x, d_y, partials, ntiles, s)
+                           : launch_tiled<true, false>(op, t, nvec, d_x, d_y, partials, ntiles, s);
+    else rc = narrow ? launch_tiled<false, true>(op, t, nvec, d_x, d_y, partials, ntiles, s)
+                     : launch_tiled<false, false>(op, t, nvec, d_x, d_y, partials, ntiles, s);
+    if (rc) return rc;
     if (d_expect) {
-        hipLaunchKernelGGL(k_reduce_partials, dim3(nvec), dim3(256), 0, s, uint32_t(nblocks), nvec,
-                           partials, d_expect);
+        hipLaunchKernelGGL(k_reduce_partials, dim3(nvec), dim3(256), 0, s, ntiles, partials,
+                           d_expect);
         QR_HIP_CHECK(hipGetLastError());
     }
     return QR_OK;

@@ -10,6 +10,8 @@
 
 #include "../../include/qrhip.h"
 
+struct qr_op;
+
 namespace qr {
 
 // Sets the thread's last error and returns `code`.
@@ -43,7 +45,27 @@ struct DeviceTables {
     uint8_t *gkind = nullptr;    // [G]   0 complex, 1 real, 2 imaginary coefficients
     double *partials = nullptr;  // workspace for <x|y> block partials
     size_t partials_cap = 0;     // in doubles
+    // matrix-free apply: terms of each group merged by their z bits inside a
+    // tile of 2^L rows ("patterns"), one table set per tile size
+    struct Patterns {
+        int L = -1;
+        uint32_t P = 0;
+        uint32_t *gpat = nullptr;   // [G+1] pattern range of each group
+        uint32_t *pzl = nullptr;    // [P]   z & (2^L - 1) of the pattern
+        uint32_t *ptptr = nullptr;  // [P+1] term range of each pattern
+        uint64_t *ptz = nullptr;    // [T]   z masks in pattern order
+        double2 *ptc = nullptr;     // [T]   c' in pattern order
+    } pat[2];
 };
+
+// Host build of the pattern tables for tile size L (see qr_apply.hip).
+struct HostPatterns {
+    std::vector<uint32_t> gpat, pzl, ptptr;
+    std::vector<uint64_t> ptz;
+    std::vector<double> ptc;  // 2 per term
+};
+void build_patterns(const qr_op *op, int L, HostPatterns &out);
+int device_patterns(const qr_op *op, DeviceTables *t, int slot, int L);
 
 }  // namespace qr
 
