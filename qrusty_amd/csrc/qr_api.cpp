@@ -103,7 +103,7 @@ static void free_tables(DeviceTables &t) {
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &pt : t.pat) {
-        void *pp[] = {pt.gpat, pt.pzl, pt.ptptr, pt.ptz, pt.ptc};
+        void *pp[] = {pt.gpat, pt.gcls, pt.gU, pt.ug, pt.pzl, pt.ptptr, pt.ptz, pt.ptc};
         for (void *p : pp)
             if (p) (void)hipFree(p);
     }
@@ -113,14 +113,17 @@ static void free_tables(DeviceTables &t) {
 // Patterns: inside a tile of 2^L rows (fixed high bits h, low bits l), term t
 // of group g contributes (-1)^popcount(h & z_t) * c'_t * (-1)^popcount(l & zl_t)
 // with zl_t = z_t & (2^L - 1).  Terms of a group that share zl_t therefore
-// merge, per tile, into one coefficient.  Patterns keep first-occurrence order
-// and their terms keep input order.
-void build_patterns(const qr_op *op, int L, HostPatterns &out) {
+// merge, per tile, into one coefficient.  The kernel splits l into tb
+// thread-index bits and the row-index bits above them; a pattern whose zl
+// lies in the thread bits costs once per thread, one whose zl lies in the
+// row bits once per block (a small per-tile table), and only the straddling
+// ones cost per row.  Terms keep input order inside a pattern.
+void build_patterns(const qr_op *op, int L, int tb, HostPatterns &out) {
     const uint64_t mask = L >= 64 ? ~uint64_t(0) : ((uint64_t(1) << L) - 1);
+    const uint64_t tmask = (uint64_t(1) << tb) - 1;
     out = HostPatterns{};
     out.gpat.push_back(0);
     out.ptptr.push_back(0);
-    std::vector<uint32_t> order;
     for (uint64_t g = 0; g < op->G; ++g) {
         std::vector<uint64_t> zls;
         std::vector<std::vector<uint32_t>> members;
@@ -131,16 +134,35 @@ void build_patterns(const qr_op *op, int L, HostPatterns &out) {
             if (k == zls.size()) { zls.push_back(zl); members.emplace_back(); }
             members[k].push_back(i);
         }
-        for (size_t k = 0; k < zls.size(); ++k) {
-            out.pzl.push_back(uint32_t(zls[k]));
-            for (uint32_t i : members[k]) {
-                out.ptz.push_back(op->tz[i]);
-                out.ptc.push_back(op->tc[2 * i]);
-                out.ptc.push_back(op->tc[2 * i + 1]);
+        auto cls = [tmask](uint64_t zl) {
+            if ((zl & ~tmask) == 0) return 0;   // thread bits only (or zl == 0)
+            if ((zl & tmask) == 0) return 2;    // row-index bits only
+            return 1;                           // straddling
+        };
+        uint32_t starts[3] = {0, 0, 0};
+        for (int c = 0; c < 3; ++c) {
+            starts[c] = uint32_t(out.pzl.size());
+            for (size_t k = 0; k < zls.size(); ++k) {
+                if (cls(zls[k]) != c) continue;
+                out.pzl.push_back(uint32_t(zls[k]));
+                for (uint32_t i : members[k]) {
+                    out.ptz.push_back(op->tz[i]);
+                    out.ptc.push_back(op->tc[2 * i]);
+                    out.ptc.push_back(op->tc[2 * i + 1]);
+                }
+                out.ptptr.push_back(uint32_t(out.ptz.size()));
             }
-            out.ptptr.push_back(uint32_t(out.ptz.size()));
         }
-        out.gpat.push_back(uint32_t(out.pzl.size()));
+        out.gcls.push_back(starts[1]);
+        out.gcls.push_back(starts[2]);
+        const uint32_t end = uint32_t(out.pzl.size());
+        if (end > starts[2]) {
+            out.gU.push_back(int32_t(out.ug.size()));
+            out.ug.push_back(uint32_t(g));
+        } else {
+            out.gU.push_back(-1);
+        }
+        out.gpat.push_back(end);
     }
 }
 
@@ -176,23 +198,26 @@ int device_tables(const qr_op *op, DeviceTables **out) {
     return QR_OK;
 }
 
-int device_patterns(const qr_op *op, DeviceTables *t, int slot, int L) {
+int device_patterns(const qr_op *op, DeviceTables *t, int slot, int L, int tb) {
     std::lock_guard<std::mutex> lock(op->mu);
     DeviceTables::Patterns &pt = t->pat[slot];
     if (pt.L == L) return QR_OK;
-    for (void *p : {(void *)pt.gpat, (void *)pt.pzl, (void *)pt.ptptr, (void *)pt.ptz, (void *)pt.ptc})
+    for (void *p : {(void *)pt.gpat, (void *)pt.gcls, (void *)pt.gU, (void *)pt.ug, (void *)pt.pzl,
+                    (void *)pt.ptptr, (void *)pt.ptz, (void *)pt.ptc})
         if (p) (void)hipFree(p);
     pt = DeviceTables::Patterns{};
     HostPatterns hp;
-    build_patterns(op, L, hp);
+    build_patterns(op, L, tb, hp);
     std::vector<double2> c2(hp.ptz.size());
     for (size_t i = 0; i < c2.size(); ++i) c2[i] = make_double2(hp.ptc[2 * i], hp.ptc[2 * i + 1]);
     int rc;
-    if ((rc = upload(&pt.gpat, hp.gpat)) || (rc = upload(&pt.pzl, hp.pzl)) ||
-        (rc = upload(&pt.ptptr, hp.ptptr)) || (rc = upload(&pt.ptz, hp.ptz)) ||
-        (rc = upload(&pt.ptc, c2)))
+    if ((rc = upload(&pt.gpat, hp.gpat)) || (rc = upload(&pt.gcls, hp.gcls)) ||
+        (rc = upload(&pt.gU, hp.gU)) || (rc = upload(&pt.ug, hp.ug)) ||
+        (rc = upload(&pt.pzl, hp.pzl)) || (rc = upload(&pt.ptptr, hp.ptptr)) ||
+        (rc = upload(&pt.ptz, hp.ptz)) || (rc = upload(&pt.ptc, c2)))
         return rc;
     pt.P = uint32_t(hp.pzl.size());
+    pt.nU = uint32_t(hp.ug.size());
     pt.L = L;
     return QR_OK;
 }

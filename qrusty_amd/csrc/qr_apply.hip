@@ -47,7 +47,7 @@ namespace qr {
 
 constexpr int kApplyThreads = 1024;
 constexpr int kApplyThreadBits = 10;
-constexpr int kCompBlock = 8;       // groups per FP32 partial between TwoSum steps
+constexpr int kCompBlock = 13;      // groups per FP32 partial between TwoSum steps
 constexpr uint32_t kMaxPatLDS = 4096;  // per-tile pattern coefficients kept in LDS
 
 template <bool kMixed>
@@ -79,42 +79,70 @@ __device__ __forceinline__ double wave_sum(double v) {
 // One group for the R rows of a thread: D_g(r) from the tile's pattern
 // coefficients, then the gather of x[r ^ x_g] and the product into the
 // accumulators (FP64: ar/ai; mixed: the FP32 block partial br/bi).
-template <bool kMixed, bool kReal, int R, typename cT>
+// With the coefficients in LDS (kPatLDS) the patterns are taken by class:
+// thread-bit patterns once per thread, row-bit patterns from the per-tile
+// table sU, straddling patterns per row.  Otherwise every pattern is formed
+// on the fly and applied per row.
+template <bool kMixed, bool kReal, bool kFull, bool kPatLDS, int R, typename cT>
 __device__ __forceinline__ void apply_group(
     uint32_t g, uint32_t tid, uint64_t tile_rows, uint64_t base, const uint64_t *__restrict__ gx,
-    const uint32_t *__restrict__ gpat, const uint32_t *__restrict__ pzl,
+    const uint32_t *__restrict__ gpat, const uint32_t *__restrict__ gcls,
+    const int32_t *__restrict__ gU, const uint32_t *__restrict__ pzl,
     const uint32_t *__restrict__ ptptr, const uint64_t *__restrict__ ptz,
-    const double2 *__restrict__ ptc, bool pat_lds, const double2 *sA, const cT *xt,
+    const double2 *__restrict__ ptc, const double2 *sA, const double2 *sU, const cT *xt,
     const cT *__restrict__ xv, double (&ar)[R], double (&ai)[R], float (&br)[R],
     float (&bi)[R]) {
     const uint64_t m = gx[g];
     double dr[R], di[R];
-#pragma unroll
-    for (int k = 0; k < R; ++k) dr[k] = di[k] = 0.0;
-    const uint32_t p1 = gpat[g + 1];
-    for (uint32_t p = gpat[g]; p < p1; ++p) {
-        // tile coefficient of the pattern (wave-uniform)
-        double a_r = 0.0, a_i = 0.0;
-        if (pat_lds) {
+    const uint32_t p0 = gpat[g], p1 = gpat[g + 1];
+    if (kPatLDS) {
+        const uint32_t ps = gcls[2 * g], pk = gcls[2 * g + 1];
+        // thread-bit patterns: one sign per thread
+        double cr = 0.0, ci = 0.0;
+        for (uint32_t p = p0; p < ps; ++p) {
             const double2 a = sA[p];
-            a_r = a.x;
-            a_i = a.y;
-        } else {
+            const double sgn = (__popc(tid & pzl[p]) & 1u) ? -1.0 : 1.0;
+            cr = fma(sgn, a.x, cr);
+            if (!kReal) ci = fma(sgn, a.y, ci);
+        }
+        const int32_t u = gU[g];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            dr[k] = cr;
+            di[k] = ci;
+            if (u >= 0) {  // row-bit patterns: per-tile table
+                const double2 uk = sU[u * R + k];
+                dr[k] += uk.x;
+                if (!kReal) di[k] += uk.y;
+            }
+        }
+        // straddling patterns: per row
+        for (uint32_t p = ps; p < pk; ++p) {
+            const double2 a = sA[p];
+            const uint32_t zl = pzl[p];
+            const uint32_t st = __popc(tid & zl) & 1u;
+            const double vr = st ? -a.x : a.x, vi = st ? -a.y : a.y;
+            const uint32_t zk = zl >> kApplyThreadBits;
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                const double sg = (__popc(uint32_t(k) & zk) & 1u) ? -1.0 : 1.0;  // uniform
+                dr[k] = fma(sg, vr, dr[k]);
+                if (!kReal) di[k] = fma(sg, vi, di[k]);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < R; ++k) dr[k] = di[k] = 0.0;
+        for (uint32_t p = p0; p < p1; ++p) {
+            double a_r = 0.0, a_i = 0.0;
             const uint32_t t1 = ptptr[p + 1];
             for (uint32_t t = ptptr[p]; t < t1; ++t) {
                 const double2 c = ptc[t];
                 if (__popcll(base & ptz[t]) & 1) { a_r -= c.x; a_i -= c.y; }
                 else { a_r += c.x; a_i += c.y; }
             }
-        }
-        const uint32_t zl = pzl[p];
-        if (zl == 0) {
-#pragma unroll
-            for (int k = 0; k < R; ++k) {
-                dr[k] += a_r;
-                if (!kReal) di[k] += a_i;
-            }
-        } else {
+            // zl == 0 gives sign +1 everywhere and fma(1, a, d) == d + a
+            const uint32_t zl = pzl[p];
             const uint32_t st = __popc(tid & zl) & 1u;  // per-thread sign
             const double vr = st ? -a_r : a_r, vi = st ? -a_i : a_i;
             const uint32_t zk = zl >> kApplyThreadBits;
@@ -135,20 +163,19 @@ __device__ __forceinline__ void apply_group(
     if (mhi) {
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            const uint64_t l = tid + uint64_t(k) * kApplyThreads;
-            if (l < tile_rows) xg[k] = far[tx + ((uint32_t(k) ^ mk) << kApplyThreadBits)];
+            if (kFull || tid + uint64_t(k) * kApplyThreads < tile_rows)
+                xg[k] = far[tx + ((uint32_t(k) ^ mk) << kApplyThreadBits)];
         }
     } else {
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            const uint64_t l = tid + uint64_t(k) * kApplyThreads;
-            if (l < tile_rows) xg[k] = xt[tx + ((uint32_t(k) ^ mk) << kApplyThreadBits)];
+            if (kFull || tid + uint64_t(k) * kApplyThreads < tile_rows)
+                xg[k] = xt[tx + ((uint32_t(k) ^ mk) << kApplyThreadBits)];
         }
     }
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-        const uint64_t l = tid + uint64_t(k) * kApplyThreads;
-        if (l >= tile_rows) break;
+        if (!kFull && tid + uint64_t(k) * kApplyThreads >= tile_rows) break;
         const cT xb = xg[k];
         if (!kMixed) {
             ar[k] = fma(dr[k], xb.x, ar[k]);
@@ -170,14 +197,15 @@ __device__ __forceinline__ void apply_group(
     }
 }
 
-template <bool kMixed, bool kNarrow>
+template <bool kMixed, bool kNarrow, bool kFull, bool kPatLDS>
 __global__ __launch_bounds__(kApplyThreads) void k_apply_tiled(
     uint64_t N, uint32_t G, uint32_t P, uint32_t ntiles, uint32_t tiles_per_xcd,
     const uint64_t *__restrict__ gx, const uint8_t *__restrict__ gkind,
-    const uint32_t *__restrict__ gpat, const uint32_t *__restrict__ pzl,
-    const uint32_t *__restrict__ ptptr, const uint64_t *__restrict__ ptz,
-    const double2 *__restrict__ ptc, const void *__restrict__ xin, void *__restrict__ yout,
-    double *__restrict__ partials) {
+    const uint32_t *__restrict__ gpat, const uint32_t *__restrict__ gcls,
+    const int32_t *__restrict__ gU, const uint32_t *__restrict__ ug, uint32_t nU,
+    const uint32_t *__restrict__ pzl, const uint32_t *__restrict__ ptptr,
+    const uint64_t *__restrict__ ptz, const double2 *__restrict__ ptc,
+    const void *__restrict__ xin, void *__restrict__ yout, double *__restrict__ partials) {
     using Shape = ApplyShape<kMixed>;
     constexpr int L = Shape::L, R = Shape::R;
     using cT = typename std::conditional<kMixed, float2, double2>::type;
@@ -188,15 +216,14 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_tiled(
     const uint32_t v = blockIdx.x / ntiles;
     uint32_t tile = blockIdx.x % ntiles;
     if (tiles_per_xcd) tile = (tile % 8) * tiles_per_xcd + tile / 8;
-    const uint64_t tile_rows = N < (uint64_t(1) << L) ? N : (uint64_t(1) << L);
+    const uint64_t tile_rows = kFull ? (uint64_t(1) << L) : (N < (uint64_t(1) << L) ? N : (uint64_t(1) << L));
     const uint64_t base = uint64_t(tile) * tile_rows;
     const cT *__restrict__ xv = reinterpret_cast<const cT *>(xin) + uint64_t(v) * N;
 
     for (uint64_t i = tid; i < tile_rows; i += kApplyThreads) xt[i] = xv[base + i];
     // per-tile pattern coefficients A_p = sum_t (-1)^popcount(base & z_t) c'_t
     double2 *sA = reinterpret_cast<double2 *>(smem_raw + tile_rows * sizeof(cT));
-    const bool pat_lds = P <= kMaxPatLDS;
-    if (pat_lds) {
+    if (kPatLDS) {
         for (uint32_t p = tid; p < P; p += kApplyThreads) {
             double a_r = 0.0, a_i = 0.0;
             const uint32_t t1 = ptptr[p + 1];
@@ -209,6 +236,22 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_tiled(
         }
     }
     __syncthreads();
+    // per-tile row-bit table: sU[u*R + k] = sum over the row-bit patterns of
+    // group ug[u] of (-1)^popcount(k & (zl >> thread bits)) * A_p
+    double2 *sU = sA + P;
+    if (kPatLDS && nU) {
+        for (uint32_t i = tid; i < nU * R; i += kApplyThreads) {
+            const uint32_t u = i / R, k = i % R, gg = ug[u];
+            double s_r = 0.0, s_i = 0.0;
+            for (uint32_t p = gcls[2 * gg + 1]; p < gpat[gg + 1]; ++p) {
+                const double2 a = sA[p];
+                if (__popc(k & (pzl[p] >> kApplyThreadBits)) & 1u) { s_r -= a.x; s_i -= a.y; }
+                else { s_r += a.x; s_i += a.y; }
+            }
+            sU[i] = make_double2(s_r, s_i);
+        }
+        __syncthreads();
+    }
 
     // accumulators: FP64 (ar, ai) or FP32 running sum s, compensation c, block partial b
     double ar[R], ai[R];
@@ -223,11 +266,13 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_tiled(
     for (uint32_t g = 0; g < G; ++g) {
         // kind 1: every c' of the group is real, so D_g(r) is real
         if (gkind[g] == 1)
-            apply_group<kMixed, true, R>(g, tid, tile_rows, base, gx, gpat, pzl, ptptr, ptz, ptc,
-                                         pat_lds, sA, xt, xv, ar, ai, br, bi);
+            apply_group<kMixed, true, kFull, kPatLDS, R>(g, tid, tile_rows, base, gx, gpat, gcls,
+                                                         gU, pzl, ptptr, ptz, ptc, sA, sU, xt, xv,
+                                                         ar, ai, br, bi);
         else
-            apply_group<kMixed, false, R>(g, tid, tile_rows, base, gx, gpat, pzl, ptptr, ptz, ptc,
-                                          pat_lds, sA, xt, xv, ar, ai, br, bi);
+            apply_group<kMixed, false, kFull, kPatLDS, R>(g, tid, tile_rows, base, gx, gpat, gcls,
+                                                          gU, pzl, ptptr, ptz, ptc, sA, sU, xt, xv,
+                                                          ar, ai, br, bi);
         if (kMixed && (++since_flush == kCompBlock || g + 1 == G)) {
             since_flush = 0;
 #pragma unroll
@@ -247,7 +292,7 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_tiled(
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const uint64_t l = tid + uint64_t(k) * kApplyThreads;
-        if (l >= tile_rows) break;
+        if (!kFull && l >= tile_rows) break;
         double yr, yi;
         if (!kMixed) {
             yr = ar[k];
@@ -311,17 +356,22 @@ static int launch_tiled(const qr_op *op, DeviceTables *t, uint32_t nvec, const v
     const uint64_t N = uint64_t(1) << op->nq;
     const uint64_t tile_rows = N < (uint64_t(1) << Shape::L) ? N : (uint64_t(1) << Shape::L);
     const DeviceTables::Patterns &pt = t->pat[kMixed ? 1 : 0];
+    const bool plds = pt.P + size_t(pt.nU) * Shape::R <= kMaxPatLDS;
     const size_t shmem = size_t(tile_rows) * sizeof(cT) +
-                         (pt.P <= kMaxPatLDS ? size_t(pt.P) * sizeof(double2) : 0);
+                         (plds ? (size_t(pt.P) + size_t(pt.nU) * Shape::R) * sizeof(double2) : 0);
     const uint32_t tpx = (ntiles % 8 == 0) ? ntiles / 8 : 0;
-    auto kern = k_apply_tiled<kMixed, kNarrow>;
+    const bool full = tile_rows == (uint64_t(1) << Shape::L);
+    auto kern = full ? (plds ? k_apply_tiled<kMixed, kNarrow, true, true>
+                             : k_apply_tiled<kMixed, kNarrow, true, false>)
+                     : (plds ? k_apply_tiled<kMixed, kNarrow, false, true>
+                             : k_apply_tiled<kMixed, kNarrow, false, false>);
     QR_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem)));
     const uint64_t nblocks = uint64_t(ntiles) * nvec;
     if (nblocks > 0x7FFFFFFFull) return fail(QR_ERR_SIZE, "apply grid too large");
     hipLaunchKernelGGL(kern, dim3(unsigned(nblocks)), dim3(kApplyThreads), shmem, s, N,
-                       uint32_t(op->G), pt.P, ntiles, tpx, t->gx, t->gkind, pt.gpat, pt.pzl, pt.ptptr,
-                       pt.ptz, pt.ptc, d_x, d_y, partials);
+                       uint32_t(op->G), pt.P, ntiles, tpx, t->gx, t->gkind, pt.gpat, pt.gcls, pt.gU,
+                       pt.ug, pt.nU, pt.pzl, pt.ptptr, pt.ptz, pt.ptc, d_x, d_y, partials);
     QR_HIP_CHECK(hipGetLastError());
     return QR_OK;
 }
@@ -331,7 +381,7 @@ int launch_pauli_apply(const qr_op *op, DeviceTables *t, int32_t precision, uint
     const bool mixed = precision == QR_PREC_MIXED, narrow = op->nq <= 32;
     const int Lmax = mixed ? ApplyShape<true>::L : ApplyShape<false>::L;
     const int L = int(op->nq) < Lmax ? int(op->nq) : Lmax;
-    int rc = device_patterns(op, t, mixed ? 1 : 0, L);
+    int rc = device_patterns(op, t, mixed ? 1 : 0, L, kApplyThreadBits);
     if (rc) return rc;
     const uint64_t N = uint64_t(1) << op->nq;
     const uint32_t ntiles = uint32_t(N >> L);

@@ -49,8 +49,11 @@ struct DeviceTables {
     // tile of 2^L rows ("patterns"), one table set per tile size
     struct Patterns {
         int L = -1;
-        uint32_t P = 0;
+        uint32_t P = 0, nU = 0;
         uint32_t *gpat = nullptr;   // [G+1] pattern range of each group
+        uint32_t *gcls = nullptr;   // [2G]  first straddling / first row-index pattern
+        int32_t *gU = nullptr;      // [G]   row of the group in the per-tile k table, or -1
+        uint32_t *ug = nullptr;     // [nU]  groups that have row-index patterns
         uint32_t *pzl = nullptr;    // [P]   z & (2^L - 1) of the pattern
         uint32_t *ptptr = nullptr;  // [P+1] term range of each pattern
         uint64_t *ptz = nullptr;    // [T]   z masks in pattern order
@@ -58,14 +61,17 @@ struct DeviceTables {
     } pat[2];
 };
 
-// Host build of the pattern tables for tile size L (see qr_apply.hip).
+// Host build of the pattern tables for tile size L and tb thread-index bits
+// (see qr_apply.hip).  Patterns of a group are ordered by class: z bits only
+// among the thread-index bits, then straddling, then only row-index bits.
 struct HostPatterns {
-    std::vector<uint32_t> gpat, pzl, ptptr;
+    std::vector<uint32_t> gpat, gcls, ug, pzl, ptptr;
+    std::vector<int32_t> gU;
     std::vector<uint64_t> ptz;
     std::vector<double> ptc;  // 2 per term
 };
-void build_patterns(const qr_op *op, int L, HostPatterns &out);
-int device_patterns(const qr_op *op, DeviceTables *t, int slot, int L);
+void build_patterns(const qr_op *op, int L, int tb, HostPatterns &out);
+int device_patterns(const qr_op *op, DeviceTables *t, int slot, int L, int tb);
 
 }  // namespace qr
 

@@ -32,11 +32,14 @@ def _op(labels, coeffs):
 
 
 def _abs_bound(p, x, rows):
-    """sum_g |D_g(r)| |x[r ^ x_g]| for the given rows (componentwise error scale)."""
+    """sum_t |c'_t| |x[r ^ x_t]| for the given rows: the componentwise error
+    scale of a sum of products whose terms may be merged in any order (the
+    kernel merges D_g(r) per tile, not in the reference's term order)."""
+    rows = np.asarray(rows, dtype=np.uint64)
+    cabs = np.abs(p.cp.view(np.complex128))
     out = np.zeros((x.shape[0], len(rows)))
-    for i, r in enumerate(rows):
-        cols, vals = orc.make_row(p, int(r))
-        out[:, i] = (np.abs(vals)[None, :] * np.abs(x[:, cols.astype(np.int64)])).sum(axis=1)
+    for t in range(p.T):
+        out += cabs[t] * np.abs(x[:, (rows ^ p.x[t]).astype(np.int64)])
     return out
 
 
