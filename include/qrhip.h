@@ -164,6 +164,39 @@ enum qr_precision { QR_PREC_FP64 = 0, QR_PREC_MIXED = 1 };
 int qr_pauli_apply(const qr_op *op, int32_t precision, uint32_t nvec, const void *d_x,
                    void *d_y, double *d_expect, void *stream);
 
+/* ------------------------------------------------------------------------
+ * Batches (no counterpart in the reference, which builds and applies one
+ * operator at a time: lib.rs:573-585, accel.rs:338-370).  One call builds the
+ * CSR of every operator of the batch and one matvec y_k = A_k x_k each, in a
+ * single persistent launch recorded in a hipGraph (replayed while the buffers
+ * and tuning stay the same).  CSR entries are bit-identical to qr_csr_build;
+ * y_k is computed from the freshly built entries (FP64 FMAs).
+ *   d_indptr[k], d_indices[k], d_data[k] : CSR outputs sized as for
+ *       qr_csr_build, or NULL (all three, or the arrays themselves) to keep
+ *       only y_k -- except for operators with more than 4096 distinct x masks,
+ *       which run through the single-operator kernels and need them
+ *   d_x[k], d_y[k] : complex128 states of 2^n_k entries (n_k <= 32)
+ *   tuning : NULL = the tuned (or default) parameters
+ * qr_batch_tune times every tuning candidate on the given buffers
+ * (synchronising the stream), keeps and returns the fastest.
+ * ---------------------------------------------------------------------- */
+typedef struct qr_batch qr_batch;
+typedef struct {
+    uint32_t tiles_per_item;  /* row tiles per work item (1..4096)          */
+    uint32_t blocks_per_cu;   /* persistent blocks per compute unit (1..8)  */
+    uint32_t order;           /* 0: input order, 1: largest operator first  */
+    uint32_t reserved;
+} qr_tuning;
+
+int qr_batch_create(const qr_op *const *ops, uint32_t nops, qr_batch **out);
+int qr_batch_destroy(qr_batch *batch);
+int qr_batch_run(qr_batch *batch, uint64_t *const *d_indptr, uint64_t *const *d_indices,
+                 double *const *d_data, const double *const *d_x, double *const *d_y,
+                 const qr_tuning *tuning, void *stream);
+int qr_batch_tune(qr_batch *batch, uint64_t *const *d_indptr, uint64_t *const *d_indices,
+                  double *const *d_data, const double *const *d_x, double *const *d_y,
+                  qr_tuning *best, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

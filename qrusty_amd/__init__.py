@@ -31,7 +31,7 @@ import numpy as np
 from . import _lib as L
 from ._lib import QrError
 
-__all__ = ["Pauli", "SparsePauliOp", "SpMat", "spmat_dot_densevec", "csr_matrix", "QrError"]
+__all__ = ["Pauli", "SparsePauliOp", "SpMat", "spmat_dot_densevec", "csr_matrix", "Batch", "QrError"]
 __version__ = "0.1.0"
 
 _PRECISIONS = {"fp64": L.QR_PREC_FP64, "mixed": L.QR_PREC_MIXED}
@@ -430,3 +430,86 @@ def csr_matrix(m: SpMat):
     (shape, data, indices, indptr) = m.export()
     from scipy.sparse import csr_matrix as _csr
     return _csr((data, indices, indptr), shape=shape, dtype=complex)
+
+
+# ---------------------------------------------------------------------------
+# batches (no counterpart in the reference, which handles one operator at a time)
+# ---------------------------------------------------------------------------
+class Batch:
+    """Several operators built into CSR and applied to one state each, in one
+    graph launch of the persistent batch kernel (qr_batch_run).
+
+    Output buffers are allocated once and reused by every run(), so the
+    recorded graph is replayed; CSR outputs are kept only when keep_csr
+    (operators with more than 4096 distinct x masks always keep theirs).
+    """
+
+    def __init__(self, ops, keep_csr: bool = True):
+        torch = _require_gpu()
+        self._ops = list(ops)
+        if not self._ops:
+            raise QrError("empty batch")
+        self._h = [op._handle(L.QR_PHASE_ROWWISE) for op in self._ops]
+        n = len(self._ops)
+        arr = (ctypes.c_void_p * n)(*[h.h.value for h in self._h])
+        b = ctypes.c_void_p()
+        L.check(L.lib.qr_batch_create(arr, n, ctypes.byref(b)))
+        self._b = b
+        dev = torch.device("cuda", torch.cuda.current_device())
+        self.x, self.y = [], []
+        self._csr = []
+        for h in self._h:
+            N = 1 << h.num_qubits
+            self.x.append(torch.zeros(N, dtype=torch.complex128, device=dev))
+            self.y.append(torch.zeros(N, dtype=torch.complex128, device=dev))
+            if keep_csr or h.G > 4096:
+                self._csr.append((torch.empty(N + 1, dtype=torch.int64, device=dev),
+                                  torch.empty(h.nnz, dtype=torch.int64, device=dev),
+                                  torch.empty(h.nnz, dtype=torch.complex128, device=dev)))
+            else:
+                self._csr.append(None)
+        P = ctypes.c_void_p * n
+        self._pp = P(*[c[0].data_ptr() if c else None for c in self._csr])
+        self._pi = P(*[c[1].data_ptr() if c else None for c in self._csr])
+        self._pd = P(*[c[2].data_ptr() if c else None for c in self._csr])
+        self._px = P(*[t.data_ptr() for t in self.x])
+        self._py = P(*[t.data_ptr() for t in self.y])
+
+    def set_states(self, states) -> None:
+        torch = _torch()
+        for k, s in enumerate(states):
+            t = s if isinstance(s, torch.Tensor) else torch.from_numpy(np.asarray(s, dtype=np.complex128))
+            self.x[k].copy_(t.reshape(-1))
+
+    def run(self, tuning: dict | None = None) -> None:
+        """Build every CSR and y_k = A_k x_k (asynchronous on the current stream)."""
+        torch = _torch()
+        tu = None
+        if tuning is not None:
+            tu = L.QrTuning(int(tuning["tiles_per_item"]), int(tuning["blocks_per_cu"]),
+                            int(tuning["order"]), 0)
+        L.check(L.lib.qr_batch_run(self._b, self._pp, self._pi, self._pd, self._px, self._py,
+                                   ctypes.byref(tu) if tu is not None else None,
+                                   _stream_handle(torch)))
+
+    def tune(self) -> dict:
+        """Time every tuning candidate on these buffers and keep the fastest."""
+        torch = _torch()
+        tu = L.QrTuning()
+        L.check(L.lib.qr_batch_tune(self._b, self._pp, self._pi, self._pd, self._px, self._py,
+                                    ctypes.byref(tu), _stream_handle(torch)))
+        return {"tiles_per_item": tu.tiles_per_item, "blocks_per_cu": tu.blocks_per_cu,
+                "order": tu.order}
+
+    def csr(self, k: int):
+        """(indptr, indices, data) device tensors of operator k, or None."""
+        return self._csr[k]
+
+    def __len__(self) -> int:
+        return len(self._ops)
+
+    def __del__(self):
+        b = getattr(self, "_b", None)
+        if b is not None and b.value and L is not None and L.lib is not None:
+            L.lib.qr_batch_destroy(b)
+            self._b = None

@@ -42,7 +42,17 @@ SIGNATURES = {
     "qr_csr_build_rows": (ctypes.c_int, [vp, u64, u64, vp, vp, vp]),
     "qr_csr_matvec": (ctypes.c_int, [u64, u64, u64, vp, vp, vp, vp, vp, vp]),
     "qr_pauli_apply": (ctypes.c_int, [vp, i32, u32, vp, vp, vp, vp]),
+    "qr_batch_create": (ctypes.c_int, [ctypes.POINTER(vp), u32, ctypes.POINTER(vp)]),
+    "qr_batch_destroy": (ctypes.c_int, [vp]),
+    "qr_batch_run": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, vp, vp]),
+    "qr_batch_tune": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, vp, vp]),
 }
+
+
+class QrTuning(ctypes.Structure):
+    """qr_tuning of include/qrhip.h."""
+    _fields_ = [("tiles_per_item", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32),
+                ("order", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 class QrError(Exception):

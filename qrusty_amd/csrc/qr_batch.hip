@@ -1,0 +1,375 @@
+// qr_batch.hip -- many operators of uneven size: CSR builds plus one matvec
+// each, in one persistent launch captured in a hipGraph, with an autotuner.
+//
+// The reference builds and applies operators one at a time
+// (qrusty/src/lib.rs:573-585 then accel.rs:338-370).  Here the work of a whole
+// batch is a single list of (operator, row tile) items.  A persistent grid of
+// 1024-thread blocks takes items from one atomic counter; an item builds its
+// rows with the shared LDS-staged tile builder (qr_tile.h) -- CSR entries
+// bit-identical to the single-operator build -- and, fused, the same rows of
+// y = A x from the entries still in registers.  No item depends on another,
+// so every block leaves the loop as soon as the counter passes the end.
+// Operators whose G exceeds the LDS stage (kStageMaxG) are built and applied
+// by the single-operator kernels, recorded in the same graph.
+//
+// Tuning knobs (qr_tuning): tiles per item (work granularity), resident
+// blocks per CU (grid size), and the item order (input order or largest
+// operator first).  qr_batch_tune times every candidate on the caller's
+// buffers and keeps the fastest.
+#include "qr_internal.h"
+#include "qr_tile.h"
+
+#include <algorithm>
+#include <numeric>
+
+namespace qr {
+
+struct BatchOpDesc {
+    TileOp op;
+    uint64_t item_begin;  // first work item of this operator
+    uint64_t tiles;       // row tiles of this operator
+    uint32_t narrow;      // masks fit in 32 bits
+    uint32_t nch;         // group chunks per thread
+};
+
+// One persistent kernel per register class (mask width x chunks per thread):
+// items of that class only, taken from the class's own counter.
+template <bool kNarrow, int kChunks>
+__global__ __launch_bounds__(kStageThreads) void k_batch(const BatchOpDesc *__restrict__ ops,
+                                                         uint32_t nops, uint64_t total_items,
+                                                         uint32_t tiles_per_item,
+                                                         unsigned long long *__restrict__ counter) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ unsigned long long item_s;
+    for (;;) {
+        if (threadIdx.x == 0) item_s = atomicAdd(counter, 1ull);
+        __syncthreads();
+        const uint64_t item = item_s;
+        __syncthreads();
+        if (item >= total_items) break;
+        // operator owning the item: last k with item_begin <= item
+        uint32_t lo = 0, hi = nops;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (ops[mid].item_begin <= item) lo = mid;
+            else hi = mid;
+        }
+        const BatchOpDesc &d = ops[lo];
+        const uint64_t t0 = (item - d.item_begin) * tiles_per_item;
+        const uint64_t t1 = t0 + tiles_per_item < d.tiles ? t0 + tiles_per_item : d.tiles;
+        for (uint64_t t = t0; t < t1; ++t) {
+            build_tile_staged<kNarrow, kChunks, true>(d.op, t, smem_raw);
+            __syncthreads();  // the stage is reused by the next tile
+        }
+    }
+}
+
+// graph-safe reset of the work counter (a kernel, not a memset node)
+__global__ void k_zero_counter(unsigned long long *counter) { *counter = 0ull; }
+
+}  // namespace qr
+
+using namespace qr;
+
+struct qr_batch {
+    std::vector<const qr_op *> ops;
+    int dev = -1;
+    hipStream_t cap = nullptr;  // private capture stream (default priority)
+    BatchOpDesc *d_desc = nullptr;
+    size_t d_desc_cap = 0;
+    unsigned long long *d_counter = nullptr;
+    int ncu = 0;
+    // graph of the last (buffers, tuning)
+    bool have_graph = false;
+    std::vector<uintptr_t> graph_key;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool have_tuned = false;
+    qr_tuning tuned{};
+};
+
+namespace {
+
+// register classes: group chunks per thread in {1, 2, kStageMaxChunks}.  A
+// batch operator builds all 2^n rows, so n <= 32 and masks are 32-bit.
+constexpr int kClasses = 3;
+inline int class_of(uint32_t nch) { return nch <= 1 ? 0 : nch <= 2 ? 1 : 2; }
+
+struct Plan {
+    std::vector<BatchOpDesc> desc;            // grouped by class, in item order
+    uint32_t cls_begin[kClasses + 1] = {};    // descriptor range of each class
+    uint64_t cls_items[kClasses] = {};
+    size_t cls_shmem[kClasses] = {};
+    std::vector<uint32_t> big;  // operators built by the single-operator kernels
+};
+
+int make_plan(qr_batch *b, uint64_t *const *d_indptr, uint64_t *const *d_indices,
+              double *const *d_data, const double *const *d_x, double *const *d_y,
+              const qr_tuning &tu, Plan &plan) {
+    const uint32_t nops = uint32_t(b->ops.size());
+    std::vector<uint32_t> order(nops);
+    std::iota(order.begin(), order.end(), 0u);
+    if (tu.order == 1)  // largest operator first
+        std::stable_sort(order.begin(), order.end(), [b](uint32_t a, uint32_t c) {
+            return (b->ops[a]->G << b->ops[a]->nq) > (b->ops[c]->G << b->ops[c]->nq);
+        });
+    const uint32_t tpi = tu.tiles_per_item ? tu.tiles_per_item : 1;
+    std::vector<BatchOpDesc> per[kClasses];
+    for (uint32_t k : order) {
+        const qr_op *op = b->ops[k];
+        if (op->nq > 32) return fail(QR_ERR_SIZE, "batch operators are limited to 32 qubits");
+        if (!d_x || !d_x[k] || !d_y || !d_y[k]) return fail(QR_ERR_ARG, "null state pointer in batch");
+        const bool has_csr = d_indices && d_indices[k] && d_data && d_data[k];
+        DeviceTables *t;
+        int rc = device_tables(op, &t);
+        if (rc) return rc;
+        const uint32_t G = uint32_t(op->G);
+        if (op->G > kStageMaxG) {
+            if (!has_csr || !d_indptr || !d_indptr[k])
+                return fail(QR_ERR_ARG, "batch operators with more than 4096 distinct x masks "
+                                        "need CSR output buffers");
+            plan.big.push_back(k);
+            continue;
+        }
+        const uint32_t jw = stage_jw_log2(G);
+        uint32_t dbuf = 0;
+        const size_t sb = stage_bytes(G, jw, &dbuf);
+        const uint64_t N = uint64_t(1) << op->nq;
+        const uint32_t tl2 = 10 - jw + 4;
+        BatchOpDesc d{};
+        d.op = TileOp{t->gx, t->gterm, t->tz, t->tc, t->rl_ptr, t->rl,
+                      has_csr ? d_indices[k] : nullptr,
+                      has_csr ? reinterpret_cast<double2 *>(d_data[k]) : nullptr,
+                      d_indptr ? d_indptr[k] : nullptr,
+                      reinterpret_cast<const double2 *>(d_x[k]), reinterpret_cast<double2 *>(d_y[k]),
+                      0, N, G, jw, (G + (1u << jw) - 1) >> jw, dbuf};
+        d.tiles = ((N - 1) >> tl2) + 1;
+        d.narrow = 1;
+        d.nch = d.op.nchunks;
+        const int c = class_of(d.nch);
+        d.item_begin = plan.cls_items[c];
+        plan.cls_items[c] += (d.tiles + tpi - 1) / tpi;
+        plan.cls_shmem[c] = std::max(plan.cls_shmem[c], sb);
+        per[c].push_back(d);
+    }
+    for (int c = 0; c < kClasses; ++c) {
+        plan.cls_begin[c] = uint32_t(plan.desc.size());
+        plan.desc.insert(plan.desc.end(), per[c].begin(), per[c].end());
+    }
+    plan.cls_begin[kClasses] = uint32_t(plan.desc.size());
+    return QR_OK;
+}
+
+template <bool kNarrow, int kChunks>
+static void launch_class(qr_batch *b, const Plan &plan, int c, uint32_t grid_cap,
+                         uint32_t tpi, hipStream_t s);
+
+template <bool kNarrow, int kChunks>
+static void launch_class(qr_batch *b, const Plan &plan, int c, uint32_t grid_cap, uint32_t tpi,
+                         hipStream_t s) {
+    const uint64_t items = plan.cls_items[c];
+    if (items == 0) return;
+    const uint64_t grid = std::min<uint64_t>(grid_cap, items);
+    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, s, b->d_counter + c);
+    hipLaunchKernelGGL((k_batch<kNarrow, kChunks>), dim3(unsigned(grid)), dim3(kStageThreads),
+                       plan.cls_shmem[c], s, b->d_desc + plan.cls_begin[c],
+                       plan.cls_begin[c + 1] - plan.cls_begin[c], items, tpi, b->d_counter + c);
+}
+
+int record(qr_batch *b, const Plan &plan, const qr_tuning &tu, uint64_t *const *d_indptr,
+           uint64_t *const *d_indices, double *const *d_data, const double *const *d_x,
+           double *const *d_y, hipStream_t s) {
+    const uint32_t bpc = tu.blocks_per_cu ? tu.blocks_per_cu : 1;
+    const uint32_t cap = uint32_t(b->ncu) * bpc;
+    const uint32_t tpi = tu.tiles_per_item ? tu.tiles_per_item : 1u;
+    launch_class<true, 1>(b, plan, 0, cap, tpi, s);
+    launch_class<true, 2>(b, plan, 1, cap, tpi, s);
+    launch_class<true, kStageMaxChunks>(b, plan, 2, cap, tpi, s);
+    QR_HIP_CHECK(hipGetLastError());
+    for (uint32_t k : plan.big) {
+        const qr_op *op = b->ops[k];
+        DeviceTables *t;
+        int rc = device_tables(op, &t);
+        if (rc) return rc;
+        const uint64_t N = uint64_t(1) << op->nq;
+        rc = launch_csr_build(op, t, 0, N, d_indptr[k], d_indices[k], d_data[k], s);
+        if (rc) return rc;
+        rc = launch_csr_matvec(N, N, N * op->G, d_indptr[k], d_indices[k], d_data[k], d_x[k],
+                               d_y[k], s);
+        if (rc) return rc;
+    }
+    return QR_OK;
+}
+
+int prepare(qr_batch *b, const Plan &plan) {
+    int dev = 0;
+    QR_HIP_CHECK(hipGetDevice(&dev));
+    if (b->dev != dev) {
+        if (b->dev >= 0) return fail(QR_ERR_ARG, "a batch is bound to the device it first ran on");
+        b->dev = dev;
+        QR_HIP_CHECK(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
+        QR_HIP_CHECK(hipMalloc(&b->d_counter, kClasses * sizeof(unsigned long long)));
+        hipDeviceProp_t prop;
+        QR_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+        b->ncu = prop.multiProcessorCount;
+        const int maxsh = int(stage_bytes(kStageMaxG, 10, nullptr));
+        const void *kerns[kClasses] = {
+            reinterpret_cast<const void *>(k_batch<true, 1>),
+            reinterpret_cast<const void *>(k_batch<true, 2>),
+            reinterpret_cast<const void *>(k_batch<true, kStageMaxChunks>)};
+        for (const void *kf : kerns)
+            QR_HIP_CHECK(hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, maxsh));
+    }
+    // a previous graph or tuning pass may still read the descriptors
+    QR_HIP_CHECK(hipDeviceSynchronize());
+    if (b->d_desc_cap < plan.desc.size()) {
+        if (b->d_desc) (void)hipFree(b->d_desc);
+        b->d_desc = nullptr;
+        b->d_desc_cap = 0;
+        QR_HIP_CHECK(hipMalloc(&b->d_desc, std::max<size_t>(plan.desc.size(), 1) * sizeof(BatchOpDesc)));
+        b->d_desc_cap = plan.desc.size();
+    }
+    if (!plan.desc.empty())
+        QR_HIP_CHECK(hipMemcpy(b->d_desc, plan.desc.data(), plan.desc.size() * sizeof(BatchOpDesc),
+                               hipMemcpyHostToDevice));
+    return QR_OK;
+}
+
+void drop_graph(qr_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->have_graph = false;
+}
+
+std::vector<uintptr_t> key_of(const qr_batch *b, uint64_t *const *d_indptr,
+                              uint64_t *const *d_indices, double *const *d_data,
+                              const double *const *d_x, double *const *d_y, const qr_tuning &tu) {
+    std::vector<uintptr_t> k;
+    const size_t n = b->ops.size();
+    k.reserve(5 * n + 3);
+    for (size_t i = 0; i < n; ++i) {
+        k.push_back(d_indptr ? uintptr_t(d_indptr[i]) : 0);
+        k.push_back(d_indices ? uintptr_t(d_indices[i]) : 0);
+        k.push_back(d_data ? uintptr_t(d_data[i]) : 0);
+        k.push_back(uintptr_t(d_x[i]));
+        k.push_back(uintptr_t(d_y[i]));
+    }
+    k.push_back(tu.tiles_per_item);
+    k.push_back(tu.blocks_per_cu);
+    k.push_back(tu.order);
+    return k;
+}
+
+const qr_tuning kDefaultTuning{1, 1, 1, 0};
+
+}  // namespace
+
+extern "C" {
+
+int qr_batch_create(const qr_op *const *ops, uint32_t nops, qr_batch **out) {
+    if (!out) return fail(QR_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (!ops || nops == 0) return fail(QR_ERR_ARG, "empty batch");
+    for (uint32_t k = 0; k < nops; ++k)
+        if (!ops[k]) return fail(QR_ERR_ARG, "null operator in batch");
+    qr_batch *b = new (std::nothrow) qr_batch;
+    if (!b) return fail(QR_ERR_NOMEM, "out of host memory");
+    b->ops.assign(ops, ops + nops);
+    *out = b;
+    return QR_OK;
+}
+
+int qr_batch_destroy(qr_batch *b) {
+    if (!b) return QR_OK;
+    drop_graph(b);
+    if (b->d_desc) (void)hipFree(b->d_desc);
+    if (b->d_counter) (void)hipFree(b->d_counter);
+    if (b->cap) (void)hipStreamDestroy(b->cap);
+    delete b;
+    return QR_OK;
+}
+
+int qr_batch_run(qr_batch *b, uint64_t *const *d_indptr, uint64_t *const *d_indices,
+                 double *const *d_data, const double *const *d_x, double *const *d_y,
+                 const qr_tuning *tuning, void *stream) {
+    if (!b || !d_x || !d_y) return fail(QR_ERR_ARG, "null argument");
+    const qr_tuning tu = tuning ? *tuning : (b->have_tuned ? b->tuned : kDefaultTuning);
+    if (tu.tiles_per_item == 0 || tu.tiles_per_item > 4096 || tu.blocks_per_cu == 0 ||
+        tu.blocks_per_cu > 8 || tu.order > 1)
+        return fail(QR_ERR_ARG, "tuning out of range");
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<uintptr_t> key = key_of(b, d_indptr, d_indices, d_data, d_x, d_y, tu);
+    if (!(b->have_graph && key == b->graph_key)) {
+        Plan plan;
+        int rc = make_plan(b, d_indptr, d_indices, d_data, d_x, d_y, tu, plan);
+        if (rc) return rc;
+        rc = prepare(b, plan);
+        if (rc) return rc;
+        drop_graph(b);
+        QR_HIP_CHECK(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+        rc = record(b, plan, tu, d_indptr, d_indices, d_data, d_x, d_y, b->cap);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(b->cap, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+        b->graph = g;
+        QR_HIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        b->have_graph = true;
+        b->graph_key = key;
+    }
+    QR_HIP_CHECK(hipGraphLaunch(b->exec, s));
+    return QR_OK;
+}
+
+int qr_batch_tune(qr_batch *b, uint64_t *const *d_indptr, uint64_t *const *d_indices,
+                  double *const *d_data, const double *const *d_x, double *const *d_y,
+                  qr_tuning *best, void *stream) {
+    if (!b || !d_x || !d_y) return fail(QR_ERR_ARG, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    const uint32_t tpis[] = {1, 2, 4, 8};
+    const uint32_t bpcs[] = {1, 2};
+    const uint32_t orders[] = {0, 1};
+    hipEvent_t e0, e1;
+    QR_HIP_CHECK(hipEventCreate(&e0));
+    QR_HIP_CHECK(hipEventCreate(&e1));
+    float best_ms = -1.0f;
+    qr_tuning bt = kDefaultTuning;
+    int rc = QR_OK;
+    for (uint32_t order : orders)
+        for (uint32_t bpc : bpcs)
+            for (uint32_t tpi : tpis) {
+                const qr_tuning tu{tpi, bpc, order, 0};
+                Plan plan;
+                if ((rc = make_plan(b, d_indptr, d_indices, d_data, d_x, d_y, tu, plan)) ||
+                    (rc = prepare(b, plan)))
+                    goto done;
+                // one untimed pass (first touch), then one timed pass
+                for (int rep = 0; rep < 2; ++rep) {
+                    if (rep == 1 && hipEventRecord(e0, s) != hipSuccess) { rc = fail(QR_ERR_HIP, "event"); goto done; }
+                    if ((rc = record(b, plan, tu, d_indptr, d_indices, d_data, d_x, d_y, s))) goto done;
+                }
+                if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+                    rc = fail(QR_ERR_HIP, "event");
+                    goto done;
+                }
+                float ms = 0.0f;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                if (best_ms < 0.0f || ms < best_ms) { best_ms = ms; bt = tu; }
+            }
+done:
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    drop_graph(b);  // the descriptors now hold the last candidate's plan
+    b->tuned = bt;
+    b->have_tuned = true;
+    if (best) *best = bt;
+    return QR_OK;
+}
+
+}  // extern "C"

@@ -22,25 +22,11 @@
 // non-temporal loads, gathers x, accumulates in FP64 with FMAs and reduces the
 // team with cross-lane shuffles.
 #include "qr_internal.h"
+#include "qr_tile.h"
 
 namespace qr {
 
-typedef double dvec2 __attribute__((ext_vector_type(2)));
-
 constexpr int kBuildThreads = 256;
-constexpr int kBuildM = 16;  // rows per thread per tile (4 varying row bits)
-
-template <bool kNarrow>
-__device__ __forceinline__ uint32_t parity_and(uint64_t a, uint64_t b) {
-    if (kNarrow) return __popc(uint32_t(a) & uint32_t(b)) & 1u;
-    return __popcll(a & b) & 1u;
-}
-
-// (-1)^s * c, by flipping sign bits (exact, like the reference's -coeff)
-__device__ __forceinline__ double2 signed_coeff(double2 c, uint32_t s) {
-    if (s) { c.x = -c.x; c.y = -c.y; }
-    return c;
-}
 
 // Thread layout: a block has 256 threads = RS rows x JW groups (JW a power of
 // two >= min(G, 256)) and owns one tile of R = RS*16 aligned rows; it walks
@@ -153,121 +139,12 @@ __global__ __launch_bounds__(kBuildThreads) void k_csr_build(
 #endif
 }
 
-// LDS-staged build (used when G <= kStageMaxG).  Same thread/row layout and
-// rank decomposition as k_csr_build, but with 1024 threads (JW up to 1024
-// groups per chunk, up to kStageMaxChunks chunks per thread) and the m loop
-// outermost: each iteration produces a batch of RS consecutive rows, written
-// into LDS at their rank positions, then copied to HBM by consecutive threads
-// taking consecutive entries, so every store instruction is fully coalesced.
-// Two stage buffers alternate between batches when they fit, leaving one
-// barrier per batch; otherwise one buffer and two barriers.
-constexpr int kStageThreads = 1024;
-constexpr uint32_t kStageMaxG = 4096;
-constexpr int kStageMaxChunks = 4;
-
+// LDS-staged build (used when G <= kStageMaxG): one block per tile of the
+// shared tile builder (qr_tile.h), without the fused matvec.
 template <bool kNarrow, int kChunks>
-__global__ __launch_bounds__(kStageThreads) void k_csr_build_staged(
-    uint64_t row_begin, uint64_t row_end, uint32_t G, uint32_t jw_log2, uint32_t nchunks,
-    uint32_t dbuf, const uint64_t *__restrict__ gx, const uint32_t *__restrict__ gterm,
-    const uint64_t *__restrict__ tz, const double2 *__restrict__ tc,
-    const uint32_t *__restrict__ rl_ptr, const RankEntry *__restrict__ rl,
-    uint64_t *__restrict__ indices, double2 *__restrict__ data) {
+__global__ __launch_bounds__(kStageThreads) void k_csr_build_staged(TileOp op) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const uint32_t rs_log2 = 10u - jw_log2;
-    const uint32_t tile_log2 = rs_log2 + 4u;
-    const uint32_t tid = threadIdx.x;
-    const uint32_t lane = tid & ((1u << jw_log2) - 1u);
-    const uint32_t ro = tid >> jw_log2;
-    const uint64_t tile_base = ((row_begin >> tile_log2) + blockIdx.x) << tile_log2;
-    const uint64_t rfix = tile_base | ro;
-    const uint32_t batch = G << rs_log2;                 // entries per batch of RS rows
-    const uint32_t batch_pad = (batch + 1u) & ~1u;
-    uint64_t *st_idx[2];
-    double2 *st_dat[2];
-    st_idx[0] = reinterpret_cast<uint64_t *>(smem_raw);
-    st_dat[0] = reinterpret_cast<double2 *>(st_idx[0] + batch_pad);
-    st_idx[1] = reinterpret_cast<uint64_t *>(st_dat[0] + batch_pad);
-    st_dat[1] = reinterpret_cast<double2 *>(st_idx[1] + batch_pad);
-
-    int32_t rfx[kChunks], s0[kChunks], s1[kChunks],
-        s2[kChunks], s3[kChunks];
-    uint64_t xj[kChunks], z0[kChunks], z1[kChunks];
-    double2 c0[kChunks], c1[kChunks];
-    uint32_t t0[kChunks], t1[kChunks];
-#pragma unroll
-    for (int c = 0; c < kChunks; ++c) {
-        const uint32_t j = (uint32_t(c) << jw_log2) + lane;
-        rfx[c] = int32_t(j);
-        s0[c] = s1[c] = s2[c] = s3[c] = 0;
-        xj[c] = z0[c] = z1[c] = 0;
-        c0[c] = c1[c] = make_double2(0.0, 0.0);
-        t0[c] = t1[c] = 0;
-        if (uint32_t(c) >= nchunks || j >= G) continue;
-        const uint32_t e1 = rl_ptr[j + 1];
-        for (uint32_t e = rl_ptr[j]; e < e1; ++e) {
-            const RankEntry en = rl[e];
-            const uint32_t b = uint32_t(en.bit);
-            const uint32_t q = b - rs_log2;  // wraps for b < rs_log2
-            if (q == 0) s0[c] = en.sigma;
-            else if (q == 1) s1[c] = en.sigma;
-            else if (q == 2) s2[c] = en.sigma;
-            else if (q == 3) s3[c] = en.sigma;
-            else if ((rfix >> b) & 1) rfx[c] += en.sigma;
-        }
-        xj[c] = gx[j];
-        t0[c] = gterm[j];
-        t1[c] = gterm[j + 1];
-        z0[c] = tz[t0[c]];
-        c0[c] = tc[t0[c]];
-        if (t1[c] > t0[c] + 1) { z1[c] = tz[t0[c] + 1]; c1[c] = tc[t0[c] + 1]; }
-    }
-
-    for (int m = 0; m < kBuildM; ++m) {
-        const int buf = dbuf ? (m & 1) : 0;
-        uint64_t *si = st_idx[buf];
-        double2 *sd = st_dat[buf];
-        const uint64_t r = rfix + (uint64_t(m) << rs_log2);
-        const bool row_ok = r >= row_begin && r < row_end;
-#pragma unroll
-        for (int c = 0; c < kChunks; ++c) {
-            const uint32_t j = (uint32_t(c) << jw_log2) + lane;
-            if (uint32_t(c) >= nchunks || j >= G || !row_ok) continue;
-            const int32_t rank = rfx[c] + ((m & 1) ? s0[c] : 0) + ((m & 2) ? s1[c] : 0) +
-                                 ((m & 4) ? s2[c] : 0) + ((m & 8) ? s3[c] : 0);
-            // term-order fold: first value, then + (accel.rs:191-208)
-            double2 v = signed_coeff(c0[c], parity_and<kNarrow>(r, z0[c]));
-            if (t1[c] > t0[c] + 1) {
-                const double2 e = signed_coeff(c1[c], parity_and<kNarrow>(r, z1[c]));
-                v.x = v.x + e.x;
-                v.y = v.y + e.y;
-                for (uint32_t t = t0[c] + 2; t < t1[c]; ++t) {
-                    const double2 ct = signed_coeff(tc[t], parity_and<kNarrow>(r, tz[t]));
-                    v.x = v.x + ct.x;
-                    v.y = v.y + ct.y;
-                }
-            }
-            const uint32_t slot = (ro * G) + uint32_t(rank);
-            si[slot] = r ^ xj[c];
-            sd[slot] = v;
-        }
-        __syncthreads();
-        const uint64_t b0 = tile_base + (uint64_t(m) << rs_log2);
-        const uint64_t lo_r = b0 > row_begin ? b0 : row_begin;
-        const uint64_t b1 = b0 + (uint64_t(1) << rs_log2);
-        const uint64_t hi_r = b1 < row_end ? b1 : row_end;
-        if (lo_r < hi_r) {
-            const uint32_t count = uint32_t(hi_r - lo_r) * G;
-            const uint32_t src = uint32_t(lo_r - b0) * G;
-            const uint64_t dst = (lo_r - row_begin) * G;
-            for (uint32_t i = tid; i < count; i += kStageThreads) {
-                indices[dst + i] = si[src + i];
-                const double2 dv = sd[src + i];
-                const dvec2 vv = {dv.x, dv.y};
-                *reinterpret_cast<dvec2 *>(data + dst + i) = vv;
-            }
-        }
-        if (!dbuf) __syncthreads();
-    }
+    build_tile_staged<kNarrow, kChunks, false>(op, blockIdx.x, smem_raw);
 }
 
 __global__ __launch_bounds__(256) void k_indptr(uint64_t row_begin, uint64_t nrows, uint64_t G,
@@ -290,16 +167,16 @@ int launch_csr_build(const qr_op *op, const DeviceTables *t, uint64_t row_begin,
     if (nrows == 0) return QR_OK;
     const bool narrow = op->nq <= 32;
     if (G <= kStageMaxG) {
-        uint32_t jw = 0;
-        while ((1u << jw) < G && jw < 10) ++jw;
+        const uint32_t jw = stage_jw_log2(G);
         const uint32_t nch = (G + (1u << jw) - 1) >> jw;
         const uint32_t tl2 = 10 - jw + 4;
         const uint64_t nt = ((row_end - 1) >> tl2) - (row_begin >> tl2) + 1;
         if (nt > 0x7FFFFFFFull) return fail(QR_ERR_SIZE, "CSR build grid too large");
-        const size_t batch_pad = ((size_t(G) << (10 - jw)) + 1) & ~size_t(1);
-        const size_t one = batch_pad * (sizeof(uint64_t) + sizeof(double2));
-        const uint32_t dbuf = 2 * one <= 96 * 1024 ? 1u : 0u;
-        const size_t shmem = dbuf ? 2 * one : one;
+        uint32_t dbuf = 0;
+        const size_t shmem = stage_bytes(G, jw, &dbuf);
+        TileOp top{t->gx, t->gterm, t->tz, t->tc, t->rl_ptr, t->rl, d_indices,
+                   reinterpret_cast<double2 *>(d_data), nullptr, nullptr, nullptr,
+                   row_begin, row_end, G, jw, nch, dbuf};
         auto kst = narrow ? (nch <= 1 ? k_csr_build_staged<true, 1>
                                       : nch <= 2 ? k_csr_build_staged<true, 2>
                                                  : k_csr_build_staged<true, kStageMaxChunks>)
@@ -308,9 +185,7 @@ int launch_csr_build(const qr_op *op, const DeviceTables *t, uint64_t row_begin,
                                                  : k_csr_build_staged<false, kStageMaxChunks>);
         QR_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(kst),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem)));
-        hipLaunchKernelGGL(kst, dim3(unsigned(nt)), dim3(kStageThreads), shmem, s, row_begin,
-                           row_end, G, jw, nch, dbuf, t->gx, t->gterm, t->tz, t->tc, t->rl_ptr,
-                           t->rl, d_indices, reinterpret_cast<double2 *>(d_data));
+        hipLaunchKernelGGL(kst, dim3(unsigned(nt)), dim3(kStageThreads), shmem, s, top);
         QR_HIP_CHECK(hipGetLastError());
         return QR_OK;
     }

@@ -1,0 +1,249 @@
+// qr_tile.h -- the LDS-staged row-tile builder shared by the single-operator
+// CSR build (qr_csr.hip) and the batched build+matvec (qr_batch.hip).
+//
+// Replaces accel::rowwise::make_row (qrusty/src/accel.rs:171-210) for one tile
+// of rows.  Thread layout: 1024 threads = RS rows x JW groups (JW a power of
+// two >= min(G, 1024)); a tile is 16 batches of RS consecutive rows.  Thread
+// (ro, lane) owns groups j = c*JW + lane (c < nchunks) and rows
+// r = tile_base + m*RS + ro, m = 0..15.  All bits of r but the four m bits are
+// fixed for the thread, so each group's rank list (qr_api.cpp:
+// build_rank_lists) is walked once per tile: entries on fixed bits are summed,
+// the at most four entries on m bits form an offset table.  Each batch is
+// written into LDS at (row, rank) and copied out by consecutive threads taking
+// consecutive entries, so every global store is fully coalesced.  Values are
+// the term-order fold of (-1)^popcount(r & z_t) c'_t (first value, then +),
+// bit-identical to the reference.
+//
+// With kMatvec the tile also produces y[r] = sum_g value * x[r ^ x_g] for its
+// rows from the values still in registers (FP64 FMAs, a fixed reduction tree
+// over the JW lanes of a row), i.e. a CSR matvec fused into the build that
+// never reads the CSR back.
+#pragma once
+
+#include "qr_internal.h"
+
+#include <type_traits>
+
+namespace qr {
+
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+
+constexpr int kBuildM = 16;            // batches per tile (4 varying row bits)
+constexpr int kStageThreads = 1024;
+constexpr uint32_t kStageMaxG = 4096;  // 24 B per entry: 96 KB of LDS for one row
+constexpr int kStageMaxChunks = 4;
+
+template <bool kNarrow>
+__device__ __forceinline__ uint32_t parity_and(uint64_t a, uint64_t b) {
+    if (kNarrow) return __popc(uint32_t(a) & uint32_t(b)) & 1u;
+    return __popcll(a & b) & 1u;
+}
+
+// (-1)^s * c, by flipping sign bits (exact, like the reference's -coeff)
+__device__ __forceinline__ double2 signed_coeff(double2 c, uint32_t s) {
+    if (s) { c.x = -c.x; c.y = -c.y; }
+    return c;
+}
+
+// One operator's view for the tile builder.
+struct TileOp {
+    const uint64_t *gx;
+    const uint32_t *gterm;
+    const uint64_t *tz;
+    const double2 *tc;
+    const uint32_t *rl_ptr;
+    const RankEntry *rl;
+    uint64_t *indices;   // CSR outputs at (r - row_begin) * G; may be null
+    double2 *data;
+    uint64_t *indptr;    // indptr[r - row_begin] = r * G; may be null
+    const double2 *x;    // fused matvec input / output (kMatvec only)
+    double2 *y;
+    uint64_t row_begin, row_end;
+    uint32_t G, jw_log2, nchunks, dbuf;
+};
+
+// LDS bytes for one operator's stage: (one or two) x (index + value) x batch,
+// plus 2 x 16 wave partials for the fused matvec.
+__host__ __device__ inline size_t stage_bytes(uint32_t G, uint32_t jw_log2, uint32_t *dbuf_out) {
+    const size_t batch_pad = ((size_t(G) << (10 - jw_log2)) + 1) & ~size_t(1);
+    const size_t one = batch_pad * (sizeof(uint64_t) + sizeof(double2));
+    const uint32_t dbuf = 2 * one <= 96 * 1024 ? 1u : 0u;
+    if (dbuf_out) *dbuf_out = dbuf;
+    return (dbuf ? 2 * one : one) + 2 * 16 * sizeof(double2);
+}
+
+__host__ __device__ inline uint32_t stage_jw_log2(uint32_t G) {
+    uint32_t jw = 0;
+    while ((1u << jw) < G && jw < 10) ++jw;
+    return jw;
+}
+
+template <bool kNarrow, int kChunks, bool kMatvec>
+__device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t tile,
+                                                  unsigned char *smem_raw) {
+    // the second term of each group stays in registers when few chunks leave room
+    constexpr bool kHold2 = kChunks <= 2;
+    const uint32_t G = op.G, jw_log2 = op.jw_log2, nchunks = op.nchunks;
+    const uint64_t row_begin = op.row_begin, row_end = op.row_end;
+    const uint32_t rs_log2 = 10u - jw_log2;
+    const uint32_t tile_log2 = rs_log2 + 4u;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & ((1u << jw_log2) - 1u);
+    const uint32_t ro = tid >> jw_log2;
+    const uint64_t tile_base = ((row_begin >> tile_log2) + tile) << tile_log2;
+    const uint64_t rfix = tile_base | ro;
+    const uint32_t batch = G << rs_log2;  // entries per batch of RS rows
+    const uint32_t batch_pad = (batch + 1u) & ~1u;
+    uint64_t *st_idx[2];
+    double2 *st_dat[2];
+    st_idx[0] = reinterpret_cast<uint64_t *>(smem_raw);
+    st_dat[0] = reinterpret_cast<double2 *>(st_idx[0] + batch_pad);
+    st_idx[1] = reinterpret_cast<uint64_t *>(st_dat[0] + batch_pad);
+    st_dat[1] = reinterpret_cast<double2 *>(st_idx[1] + batch_pad);
+    double2 *red = reinterpret_cast<double2 *>(op.dbuf ? reinterpret_cast<unsigned char *>(st_idx[1]) +
+                                                             size_t(batch_pad) * 24
+                                                       : reinterpret_cast<unsigned char *>(st_idx[1]));
+
+    // rows of the tile: indptr
+    if (op.indptr) {
+        const uint64_t tr = uint64_t(1) << tile_log2;
+        for (uint64_t i = tid; i < tr; i += kStageThreads) {
+            const uint64_t r = tile_base + i;
+            if (r >= row_begin && r < row_end) op.indptr[r - row_begin] = r * G;
+            if (r + 1 == row_end) op.indptr[r + 1 - row_begin] = (r + 1) * G;
+        }
+    }
+
+    // masks of <= 32-qubit operators are held in 32 bits
+    using mask_t = typename std::conditional<kNarrow, uint32_t, uint64_t>::type;
+    int32_t rfx[kChunks], s0[kChunks], s1[kChunks], s2[kChunks], s3[kChunks];
+    mask_t xj[kChunks], z0[kChunks], z1[kChunks];
+    double2 c0[kChunks], c1[kChunks];
+    uint32_t t0[kChunks], t1[kChunks];
+#pragma unroll
+    for (int c = 0; c < kChunks; ++c) {
+        const uint32_t j = (uint32_t(c) << jw_log2) + lane;
+        rfx[c] = int32_t(j);
+        s0[c] = s1[c] = s2[c] = s3[c] = 0;
+        xj[c] = z0[c] = z1[c] = 0;
+        c0[c] = c1[c] = make_double2(0.0, 0.0);
+        t0[c] = t1[c] = 0;
+        if (uint32_t(c) >= nchunks || j >= G) continue;
+        const uint32_t e1 = op.rl_ptr[j + 1];
+        for (uint32_t e = op.rl_ptr[j]; e < e1; ++e) {
+            const RankEntry en = op.rl[e];
+            const uint32_t b = uint32_t(en.bit);
+            const uint32_t q = b - rs_log2;  // wraps for b < rs_log2
+            if (q == 0) s0[c] = en.sigma;
+            else if (q == 1) s1[c] = en.sigma;
+            else if (q == 2) s2[c] = en.sigma;
+            else if (q == 3) s3[c] = en.sigma;
+            else if ((rfix >> b) & 1) rfx[c] += en.sigma;
+        }
+        xj[c] = mask_t(op.gx[j]);
+        t0[c] = op.gterm[j];
+        t1[c] = op.gterm[j + 1];
+        z0[c] = mask_t(op.tz[t0[c]]);
+        c0[c] = op.tc[t0[c]];
+        if (kHold2 && t1[c] > t0[c] + 1) {
+            z1[c] = mask_t(op.tz[t0[c] + 1]);
+            c1[c] = op.tc[t0[c] + 1];
+        }
+    }
+
+#pragma unroll 1
+    for (int m = 0; m < kBuildM; ++m) {
+        const int buf = op.dbuf ? (m & 1) : 0;
+        uint64_t *si = st_idx[buf];
+        double2 *sd = st_dat[buf];
+        const uint64_t r = rfix + (uint64_t(m) << rs_log2);
+        const bool row_ok = r >= row_begin && r < row_end;
+#pragma unroll
+        for (int c = 0; c < kChunks; ++c) {
+            const uint32_t j = (uint32_t(c) << jw_log2) + lane;
+            if (uint32_t(c) >= nchunks || j >= G || !row_ok) continue;
+            const int32_t rank = rfx[c] + ((m & 1) ? s0[c] : 0) + ((m & 2) ? s1[c] : 0) +
+                                 ((m & 4) ? s2[c] : 0) + ((m & 8) ? s3[c] : 0);
+            // term-order fold: first value, then + (accel.rs:191-208)
+            double2 v = signed_coeff(c0[c], parity_and<kNarrow>(r, z0[c]));
+            if (t1[c] > t0[c] + 1) {
+                const double2 e = kHold2 ? signed_coeff(c1[c], parity_and<kNarrow>(r, z1[c]))
+                                         : signed_coeff(op.tc[t0[c] + 1],
+                                                        parity_and<kNarrow>(r, op.tz[t0[c] + 1]));
+                v.x = v.x + e.x;
+                v.y = v.y + e.y;
+                for (uint32_t t = t0[c] + 2; t < t1[c]; ++t) {
+                    const double2 ct = signed_coeff(op.tc[t], parity_and<kNarrow>(r, op.tz[t]));
+                    v.x = v.x + ct.x;
+                    v.y = v.y + ct.y;
+                }
+            }
+            const uint64_t col = r ^ uint64_t(xj[c]);
+            const uint32_t slot = (ro * G) + uint32_t(rank);
+            si[slot] = col;
+            sd[slot] = v;
+        }
+        __syncthreads();
+        const uint64_t b0 = tile_base + (uint64_t(m) << rs_log2);
+        const uint64_t lo_r = b0 > row_begin ? b0 : row_begin;
+        const uint64_t b1 = b0 + (uint64_t(1) << rs_log2);
+        const uint64_t hi_r = b1 < row_end ? b1 : row_end;
+        if (kMatvec) {
+            // y of row ro from the staged entries, in column order: lane takes
+            // positions lane, lane + JW, ...; then a fixed tree over the lanes
+            double2 acc = make_double2(0.0, 0.0);
+            const uint64_t rr = b0 + ro;
+            if (rr >= row_begin && rr < row_end) {
+                for (uint32_t i = lane; i < G; i += (1u << jw_log2)) {
+                    const uint64_t col = si[ro * G + i];
+                    const double2 v = sd[ro * G + i];
+                    const double2 xb = op.x[col];
+                    acc.x = fma(v.x, xb.x, acc.x);
+                    acc.x = fma(-v.y, xb.y, acc.x);
+                    acc.y = fma(v.x, xb.y, acc.y);
+                    acc.y = fma(v.y, xb.x, acc.y);
+                }
+            }
+            const uint32_t wl = jw_log2 < 6 ? jw_log2 : 6;
+            for (uint32_t off = (1u << wl) >> 1; off > 0; off >>= 1) {
+                acc.x += __shfl_xor(acc.x, int(off), 64);
+                acc.y += __shfl_xor(acc.y, int(off), 64);
+            }
+            if (jw_log2 <= 6) {
+                if (lane == 0 && rr >= row_begin && rr < row_end) op.y[rr - row_begin] = acc;
+            } else if ((tid & 63) == 0) {
+                red[buf * 16 + (tid >> 6)] = acc;
+            }
+        }
+        if (op.indices && lo_r < hi_r) {
+            const uint32_t count = uint32_t(hi_r - lo_r) * G;
+            const uint32_t src = uint32_t(lo_r - b0) * G;
+            const uint64_t dst = (lo_r - row_begin) * G;
+            for (uint32_t i = tid; i < count; i += kStageThreads) {
+                op.indices[dst + i] = si[src + i];
+                const double2 dv = sd[src + i];
+                const dvec2 vv = {dv.x, dv.y};
+                *reinterpret_cast<dvec2 *>(op.data + dst + i) = vv;
+            }
+        }
+        if (kMatvec && jw_log2 > 6) {
+            // rows whose JW lanes span several waves: add the wave partials
+            __syncthreads();
+            if (tid < (1u << rs_log2)) {
+                const uint32_t wpr = 1u << (jw_log2 - 6);
+                double2 sum = make_double2(0.0, 0.0);
+                for (uint32_t w = 0; w < wpr; ++w) {
+                    const double2 p = red[buf * 16 + tid * wpr + w];
+                    sum.x += p.x;
+                    sum.y += p.y;
+                }
+                const uint64_t rr = b0 + tid;
+                if (rr >= row_begin && rr < row_end) op.y[rr - row_begin] = sum;
+            }
+        } else if (!op.dbuf) {
+            __syncthreads();
+        }
+    }
+}
+
+}  // namespace qr
