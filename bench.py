@@ -42,8 +42,9 @@ def load_traffic(kernel: str, workload: str):
     try:
         with open(path) as f:
             t = json.load(f)
-        return t.get(f"{kernel}@{workload}")
-    except (OSError, ValueError):
+        e = t.get(f"{kernel}@{workload}")
+        return None if e is None else e["traffic_bytes"]
+    except (OSError, ValueError, KeyError, TypeError):
         return None
 
 
@@ -297,6 +298,28 @@ def bench_batch(args, torch, qa):
     return res
 
 
+def bench_copy(args, torch):
+    """Plain device-to-device copy of 2 GiB (torch copy_): the calibration
+    kernel whose known bytes scale the HBM counters FETCH_SIZE / WRITE_SIZE."""
+    n = 1 << 28  # float64 elements = 2 GiB
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(args.warmup):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / args.steps
+    res = {"workload": "device copy 2 GiB", "bytes_read": 8 * n, "bytes_written": 8 * n,
+           "ms": ms, "GBps": 16 * n / ms / 1e6}
+    log(f"copy {ms:.3f} ms ({res['GBps']:.0f} GB/s)")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -305,7 +328,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the Heisenberg apply workload")
     ap.add_argument("--no-check", action="store_true", help="skip the sampled parity check")
-    ap.add_argument("--only", choices=["csr", "apply", "batch"], default=None)
+    ap.add_argument("--only", choices=["csr", "apply", "batch", "copy"], default=None)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
@@ -318,6 +341,10 @@ def main():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
     torch.cuda.set_device(0)
+
+    if args.only == "copy":
+        print(json.dumps({"extra": {"copy": bench_copy(args, torch)}}), flush=True)
+        return
 
     out = {"metric": "CSR nnz/s + Pauli-sum matvec GB/s (% HBM peak); FP64 vs mixed @<=1e-6 rel",
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
