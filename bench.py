@@ -221,6 +221,16 @@ def bench_apply(args, torch, qa):
     E64 = e64.cpu().numpy().view(np.complex128)
     E32 = e32.cpu().numpy().view(np.complex128)
     rE = float(np.max(np.abs(E32 - E64) / np.abs(E64)))
+    # the same FP64 path on the complex64-rounded states: separates the kernel's
+    # arithmetic error from the rounding of psi itself (|E| of a traceless H in
+    # a random state is small, so the latter dominates the relative error)
+    xr = x32.to(torch.complex128)
+    er = torch.empty_like(e64)
+    L.check(L.lib.qr_pauli_apply(h.h, L.QR_PREC_FP64, nvec, P(xr), P(y64), P(er), sh))
+    torch.cuda.synchronize()
+    ER = er.cpu().numpy().view(np.complex128)
+    rE_kernel = float(np.max(np.abs(E32 - ER) / np.abs(ER)))
+    rE_input = float(np.max(np.abs(ER - E64) / np.abs(E64)))
     res = {
         "workload": "heisenberg ring n=24, 8 states, matrix-free apply + <psi|H|psi>",
         "G": h.G, "T": h.T,
@@ -228,10 +238,13 @@ def bench_apply(args, torch, qa):
         "mixed_ms": ms32, "mixed_GBps": b32 / ms32 / 1e6, "mixed_pct_hbm": 100 * b32 / ms32 / 1e6 / HBM_PEAK_GBPS,
         "mixed_speedup_vs_fp64": ms64 / ms32,
         "mixed_rel_err_y_vs_fp64": ry, "mixed_rel_err_expect_vs_fp64": rE,
+        "mixed_rel_err_expect_vs_fp64_same_rounded_input": rE_kernel,
+        "expect_rel_change_from_rounding_psi_to_c64": rE_input,
         "algorithmic_bytes": {"fp64": b64, "mixed": b32},
     }
     log(f"apply fp64 {ms64:.3f} ms ({res['fp64_GBps']:.0f} GB/s) mixed {ms32:.3f} ms "
-        f"({res['mixed_GBps']:.0f} GB/s) speedup {ms64 / ms32:.2f}x err y {ry:.2e} E {rE:.2e}")
+        f"({res['mixed_GBps']:.0f} GB/s) speedup {ms64 / ms32:.2f}x err y {ry:.2e} E {rE:.2e} "
+        f"(kernel only {rE_kernel:.2e}, from rounding psi {rE_input:.2e})")
     return res
 
 
@@ -298,6 +311,69 @@ def bench_batch(args, torch, qa):
     return res
 
 
+def bench_large_apply(args, torch, qa):
+    """28 qubits, 10000 k-local terms (weights 1..4, real N(0,1) coefficients),
+    one 4 GiB complex128 state: mixed apply + <psi|H|psi> (the headline form
+    of this case) and the FP64 apply for reference."""
+    import numpy as np
+    from qrusty_amd import _lib as L
+    from qrusty_amd import workloads
+
+    nq, T = 28, 10000
+    labels, coeffs = workloads.k_local_op(nq, T, seed=workloads.SEED_CFG5)
+    op = qa.SparsePauliOp([qa.Pauli(l) for l in labels], coeffs)
+    h = op._handle(L.QR_PHASE_ROWWISE)
+    N = 1 << nq
+    g = torch.Generator(device="cuda")
+    g.manual_seed(workloads.SEED_CFG5 + 1)
+    x64 = torch.randn(N, dtype=torch.complex128, device="cuda", generator=g)
+    x64 /= torch.linalg.vector_norm(x64)
+    x32 = x64.to(torch.complex64)
+    y64, y32 = torch.empty_like(x64), torch.empty_like(x32)
+    e64 = torch.empty(2, dtype=torch.float64, device="cuda")
+    e32 = torch.empty(2, dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream()
+    sh = ctypes.c_void_p(stream.cuda_stream)
+    P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+
+    def timed(prec, xt, yt, et, reps):
+        L.check(L.lib.qr_pauli_apply(h.h, prec, 1, P(xt), P(yt), P(et), sh))
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        a.record(stream)
+        for _ in range(reps):
+            L.check(L.lib.qr_pauli_apply(h.h, prec, 1, P(xt), P(yt), P(et), sh))
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+    reps = max(2, min(args.steps, 5))
+    ms32 = timed(L.QR_PREC_MIXED, x32, y32, e32, reps)
+    ms64 = timed(L.QR_PREC_FP64, x64, y64, e64, reps)
+    ny = torch.linalg.vector_norm(y64).item()
+    ry = torch.linalg.vector_norm(y32.to(torch.complex128) - y64).item() / ny
+    E64 = complex(*e64.cpu().numpy())
+    E32 = complex(*e32.cpu().numpy())
+    b32, b64 = N * 16, N * 32
+    res = {
+        "workload": "k-local n=28 T=10000 (weights 1..4), 1 state: mixed apply + <psi|H|psi>",
+        "T": h.T, "G": h.G, "mixed_ms": ms32, "fp64_ms": ms64,
+        "mixed_GBps": b32 / ms32 / 1e6, "mixed_pct_hbm": 100 * b32 / ms32 / 1e6 / HBM_PEAK_GBPS,
+        "fp64_GBps": b64 / ms64 / 1e6, "mixed_speedup_vs_fp64": ms64 / ms32,
+        "sign_evals_per_s_mixed": N * h.T / (ms32 / 1e3),
+        "group_macs_per_s_mixed": N * h.G / (ms32 / 1e3),
+        "mixed_rel_err_y_vs_fp64": ry,
+        "expect_fp64": [E64.real, E64.imag], "expect_mixed": [E32.real, E32.imag],
+        "mixed_abs_err_expect": abs(E32 - E64),
+        "mixed_err_expect_over_norm_Hpsi": abs(E32 - E64) / ny,
+    }
+    log(f"k-local 28q: mixed {ms32:.1f} ms ({res['mixed_GBps']:.0f} GB/s), fp64 {ms64:.1f} ms, "
+        f"err y {ry:.2e}, |dE| {abs(E32 - E64):.2e} (|E| {abs(E64):.2e}, ||Hpsi|| {ny:.2e})")
+    del x64, x32, y64, y32
+    torch.cuda.empty_cache()
+    return res
+
+
 def bench_copy(args, torch):
     """Plain device-to-device copy of 2 GiB (torch copy_): the calibration
     kernel whose known bytes scale the HBM counters FETCH_SIZE / WRITE_SIZE."""
@@ -328,7 +404,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the Heisenberg apply workload")
     ap.add_argument("--no-check", action="store_true", help="skip the sampled parity check")
-    ap.add_argument("--only", choices=["csr", "apply", "batch", "copy"], default=None)
+    ap.add_argument("--only", choices=["csr", "apply", "batch", "large", "copy"], default=None)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
@@ -369,6 +445,8 @@ def main():
                         "dtype": "c64/fp32+TwoSum", "config": {"workload": a["workload"]}})
     if args.only in (None, "batch") and not args.no_extra:
         extra["batch"] = bench_batch(args, torch, qa)
+    if args.only in (None, "large") and not args.no_extra:
+        extra["large_apply"] = bench_large_apply(args, torch, qa)
     out["extra"] = extra
     print(json.dumps(out), flush=True)
 

@@ -103,7 +103,7 @@ static void free_tables(DeviceTables &t) {
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &pt : t.pat) {
-        void *pp[] = {pt.gpat, pt.gcls, pt.gU, pt.ug, pt.pzl, pt.ptptr, pt.ptz, pt.ptc};
+        void *pp[] = {pt.grp, pt.chunk, pt.ug, pt.pzl, pt.ptptr, pt.ptz, pt.ptc};
         for (void *p : pp)
             if (p) (void)hipFree(p);
     }
@@ -118,12 +118,29 @@ static void free_tables(DeviceTables &t) {
 // lies in the thread bits costs once per thread, one whose zl lies in the
 // row bits once per block (a small per-tile table), and only the straddling
 // ones cost per row.  Terms keep input order inside a pattern.
-void build_patterns(const qr_op *op, int L, int tb, HostPatterns &out) {
+void build_patterns(const qr_op *op, int L, int tb, uint32_t pcap, uint32_t ucap,
+                    HostPatterns &out) {
     const uint64_t mask = L >= 64 ? ~uint64_t(0) : ((uint64_t(1) << L) - 1);
     const uint64_t tmask = (uint64_t(1) << tb) - 1;
     out = HostPatterns{};
-    out.gpat.push_back(0);
     out.ptptr.push_back(0);
+    ApplyChunk cur{};
+    auto close_chunk = [&out, &cur]() {
+        cur.g1 = uint32_t(out.grp.size());
+        cur.p1 = uint32_t(out.pzl.size());
+        cur.u1 = uint32_t(out.ug.size());
+        if (cur.g1 > cur.g0) {
+            out.chunk.push_back(cur);
+            out.max_p = std::max(out.max_p, cur.p1 - cur.p0);
+            out.max_u = std::max(out.max_u, cur.u1 - cur.u0);
+        }
+        cur = ApplyChunk{cur.g1, 0, cur.p1, 0, cur.u1, 0, 0, 0};
+    };
+    auto cls = [tmask](uint64_t zl) {
+        if ((zl & ~tmask) == 0) return 0;  // thread bits only (or zl == 0)
+        if ((zl & tmask) == 0) return 2;   // row-index bits only
+        return 1;                          // straddling
+    };
     for (uint64_t g = 0; g < op->G; ++g) {
         std::vector<uint64_t> zls;
         std::vector<std::vector<uint32_t>> members;
@@ -134,36 +151,43 @@ void build_patterns(const qr_op *op, int L, int tb, HostPatterns &out) {
             if (k == zls.size()) { zls.push_back(zl); members.emplace_back(); }
             members[k].push_back(i);
         }
-        auto cls = [tmask](uint64_t zl) {
-            if ((zl & ~tmask) == 0) return 0;   // thread bits only (or zl == 0)
-            if ((zl & tmask) == 0) return 2;    // row-index bits only
-            return 1;                           // straddling
-        };
-        uint32_t starts[3] = {0, 0, 0};
-        for (int c = 0; c < 3; ++c) {
-            starts[c] = uint32_t(out.pzl.size());
-            for (size_t k = 0; k < zls.size(); ++k) {
-                if (cls(zls[k]) != c) continue;
+        std::vector<size_t> order;  // patterns by class, first-seen order inside a class
+        for (int c = 0; c < 3; ++c)
+            for (size_t k = 0; k < zls.size(); ++k)
+                if (cls(zls[k]) == c) order.push_back(k);
+        for (size_t s0 = 0; s0 < order.size(); s0 += pcap) {
+            const size_t s1 = std::min(order.size(), s0 + pcap);
+            uint32_t ncls[3] = {0, 0, 0};
+            for (size_t s = s0; s < s1; ++s) ++ncls[cls(zls[order[s]])];
+            const uint32_t n = uint32_t(s1 - s0), has_u = ncls[2] ? 1u : 0u;
+            if ((uint32_t(out.pzl.size()) - cur.p0) + n > pcap ||
+                (uint32_t(out.ug.size()) - cur.u0) + has_u > ucap)
+                close_chunk();
+            ApplyGroup a{};
+            a.m = op->gx[g];
+            a.p0 = uint32_t(out.pzl.size()) - cur.p0;
+            a.ps = a.p0 + ncls[0];
+            a.pk = a.ps + ncls[1];
+            a.p1 = a.pk + ncls[2];
+            bool real = true;
+            for (size_t s = s0; s < s1; ++s) {
+                const size_t k = order[s];
                 out.pzl.push_back(uint32_t(zls[k]));
                 for (uint32_t i : members[k]) {
                     out.ptz.push_back(op->tz[i]);
                     out.ptc.push_back(op->tc[2 * i]);
                     out.ptc.push_back(op->tc[2 * i + 1]);
+                    real = real && op->tc[2 * i + 1] == 0.0;
                 }
                 out.ptptr.push_back(uint32_t(out.ptz.size()));
             }
+            a.u = has_u ? int32_t(uint32_t(out.ug.size()) - cur.u0) : -1;
+            if (has_u) out.ug.push_back(uint32_t(out.grp.size()));
+            a.real = real ? 1u : 0u;
+            out.grp.push_back(a);
         }
-        out.gcls.push_back(starts[1]);
-        out.gcls.push_back(starts[2]);
-        const uint32_t end = uint32_t(out.pzl.size());
-        if (end > starts[2]) {
-            out.gU.push_back(int32_t(out.ug.size()));
-            out.ug.push_back(uint32_t(g));
-        } else {
-            out.gU.push_back(-1);
-        }
-        out.gpat.push_back(end);
     }
+    close_chunk();
 }
 
 template <typename T>
@@ -198,26 +222,30 @@ int device_tables(const qr_op *op, DeviceTables **out) {
     return QR_OK;
 }
 
-int device_patterns(const qr_op *op, DeviceTables *t, int slot, int L, int tb) {
+int device_patterns(const qr_op *op, DeviceTables *t, int slot, int L, int tb, uint32_t pcap,
+                    uint32_t ucap) {
     std::lock_guard<std::mutex> lock(op->mu);
     DeviceTables::Patterns &pt = t->pat[slot];
     if (pt.L == L) return QR_OK;
-    for (void *p : {(void *)pt.gpat, (void *)pt.gcls, (void *)pt.gU, (void *)pt.ug, (void *)pt.pzl,
+    for (void *p : {(void *)pt.grp, (void *)pt.chunk, (void *)pt.ug, (void *)pt.pzl,
                     (void *)pt.ptptr, (void *)pt.ptz, (void *)pt.ptc})
         if (p) (void)hipFree(p);
     pt = DeviceTables::Patterns{};
     HostPatterns hp;
-    build_patterns(op, L, tb, hp);
+    build_patterns(op, L, tb, pcap, ucap, hp);
     std::vector<double2> c2(hp.ptz.size());
     for (size_t i = 0; i < c2.size(); ++i) c2[i] = make_double2(hp.ptc[2 * i], hp.ptc[2 * i + 1]);
     int rc;
-    if ((rc = upload(&pt.gpat, hp.gpat)) || (rc = upload(&pt.gcls, hp.gcls)) ||
-        (rc = upload(&pt.gU, hp.gU)) || (rc = upload(&pt.ug, hp.ug)) ||
-        (rc = upload(&pt.pzl, hp.pzl)) || (rc = upload(&pt.ptptr, hp.ptptr)) ||
-        (rc = upload(&pt.ptz, hp.ptz)) || (rc = upload(&pt.ptc, c2)))
+    if ((rc = upload(&pt.grp, hp.grp)) || (rc = upload(&pt.chunk, hp.chunk)) ||
+        (rc = upload(&pt.ug, hp.ug)) || (rc = upload(&pt.pzl, hp.pzl)) ||
+        (rc = upload(&pt.ptptr, hp.ptptr)) || (rc = upload(&pt.ptz, hp.ptz)) ||
+        (rc = upload(&pt.ptc, c2)))
         return rc;
     pt.P = uint32_t(hp.pzl.size());
-    pt.nU = uint32_t(hp.ug.size());
+    pt.ngrp = uint32_t(hp.grp.size());
+    pt.nchunk = uint32_t(hp.chunk.size());
+    pt.max_p = hp.max_p;
+    pt.max_u = hp.max_u;
     pt.L = L;
     return QR_OK;
 }

@@ -47,13 +47,21 @@ namespace qr {
 
 constexpr int kApplyThreads = 1024;
 constexpr int kApplyThreadBits = 10;
-constexpr int kCompBlock = 13;      // groups per FP32 partial between TwoSum steps
-constexpr uint32_t kMaxPatLDS = 4096;  // per-tile pattern coefficients kept in LDS
+constexpr int kCompBlock = 16;  // groups per FP32 partial between TwoSum steps
+constexpr size_t kApplyMaxLDS = 160 * 1024;  // LDS of one gfx950 CU
 
+// Tile of 2^L rows (x tile in LDS: 64 KB complex64, 128 KB complex128), R rows
+// per thread, and the LDS left for one chunk of pattern coefficients
+// (20 B each) and the row-bit table (R x 16 B per row, plus a zero row).
 template <bool kMixed>
 struct ApplyShape {
-    static constexpr int L = kMixed ? 13 : 12;           // tile = 2^L rows
-    static constexpr int R = (1 << L) / kApplyThreads;  // rows per thread
+    static constexpr int L = kMixed ? 13 : 12;
+    static constexpr int R = (1 << L) / kApplyThreads;
+    static constexpr size_t kTile = (size_t(1) << L) * (kMixed ? 8 : 16);
+    // LDS after the x tile: 5/8 for patterns, 3/8 for the row-bit table
+    static constexpr size_t kRest = kApplyMaxLDS - kTile - 1024;
+    static constexpr uint32_t kPatChunk = uint32_t(kRest * 5 / 8 / 20) < 2048u ? uint32_t(kRest * 5 / 8 / 20) : 2048u;
+    static constexpr uint32_t kTableRows = uint32_t(kRest * 3 / 8 / (R * 16)) - 1;
 };
 
 template <bool kNarrow>
@@ -76,120 +84,108 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// One group for the R rows of a thread: D_g(r) from the tile's pattern
-// coefficients, then the gather of x[r ^ x_g] and the product into the
-// accumulators (FP64: ar/ai; mixed: the FP32 block partial br/bi).
-// With the coefficients in LDS (kPatLDS) the patterns are taken by class:
-// thread-bit patterns once per thread, row-bit patterns from the per-tile
-// table sU, straddling patterns per row.  Otherwise every pattern is formed
-// on the fly and applied per row.
-template <bool kMixed, bool kReal, bool kFull, bool kPatLDS, int R, typename cT>
-__device__ __forceinline__ void apply_group(
-    uint32_t g, uint32_t tid, uint64_t tile_rows, uint64_t base, const uint64_t *__restrict__ gx,
-    const uint32_t *__restrict__ gpat, const uint32_t *__restrict__ gcls,
-    const int32_t *__restrict__ gU, const uint32_t *__restrict__ pzl,
-    const uint32_t *__restrict__ ptptr, const uint64_t *__restrict__ ptz,
-    const double2 *__restrict__ ptc, const double2 *sA, const double2 *sU, const cT *xt,
-    const cT *__restrict__ xv, double (&ar)[R], double (&ai)[R], float (&br)[R],
-    float (&bi)[R]) {
-    const uint64_t m = gx[g];
-    double dr[R], di[R];
-    const uint32_t p0 = gpat[g], p1 = gpat[g + 1];
-    if (kPatLDS) {
-        const uint32_t ps = gcls[2 * g], pk = gcls[2 * g + 1];
-        // thread-bit patterns: one sign per thread
-        double cr = 0.0, ci = 0.0;
-        for (uint32_t p = p0; p < ps; ++p) {
-            const double2 a = sA[p];
-            const double sgn = (__popc(tid & pzl[p]) & 1u) ? -1.0 : 1.0;
-            cr = fma(sgn, a.x, cr);
-            if (!kReal) ci = fma(sgn, a.y, ci);
-        }
-        const int32_t u = gU[g];
-#pragma unroll
-        for (int k = 0; k < R; ++k) {
-            dr[k] = cr;
-            di[k] = ci;
-            if (u >= 0) {  // row-bit patterns: per-tile table
-                const double2 uk = sU[u * R + k];
-                dr[k] += uk.x;
-                if (!kReal) di[k] += uk.y;
-            }
-        }
-        // straddling patterns: per row
-        for (uint32_t p = ps; p < pk; ++p) {
-            const double2 a = sA[p];
-            const uint32_t zl = pzl[p];
-            const uint32_t st = __popc(tid & zl) & 1u;
-            const double vr = st ? -a.x : a.x, vi = st ? -a.y : a.y;
-            const uint32_t zk = zl >> kApplyThreadBits;
-#pragma unroll
-            for (int k = 0; k < R; ++k) {
-                const double sg = (__popc(uint32_t(k) & zk) & 1u) ? -1.0 : 1.0;  // uniform
-                dr[k] = fma(sg, vr, dr[k]);
-                if (!kReal) di[k] = fma(sg, vi, di[k]);
-            }
-        }
-    } else {
-#pragma unroll
-        for (int k = 0; k < R; ++k) dr[k] = di[k] = 0.0;
-        for (uint32_t p = p0; p < p1; ++p) {
-            double a_r = 0.0, a_i = 0.0;
-            const uint32_t t1 = ptptr[p + 1];
-            for (uint32_t t = ptptr[p]; t < t1; ++t) {
-                const double2 c = ptc[t];
-                if (__popcll(base & ptz[t]) & 1) { a_r -= c.x; a_i -= c.y; }
-                else { a_r += c.x; a_i += c.y; }
-            }
-            // zl == 0 gives sign +1 everywhere and fma(1, a, d) == d + a
-            const uint32_t zl = pzl[p];
-            const uint32_t st = __popc(tid & zl) & 1u;  // per-thread sign
-            const double vr = st ? -a_r : a_r, vi = st ? -a_i : a_i;
-            const uint32_t zk = zl >> kApplyThreadBits;
-#pragma unroll
-            for (int k = 0; k < R; ++k) {
-                const double sg = (__popc(uint32_t(k) & zk) & 1u) ? -1.0 : 1.0;  // uniform
-                dr[k] = fma(sg, vr, dr[k]);
-                if (!kReal) di[k] = fma(sg, vi, di[k]);
-            }
-        }
-    }
-    // gather partner rows: l ^ (m & (2^L-1)) inside tile base ^ (m >> L)
+// x(-1) when bit 31 of `flip` is set: the sign of a double flipped by an XOR
+// of its high word.
+__device__ __forceinline__ double flip_sign(double a, uint32_t flip) {
+    return __hiloint2double(__double2hiint(a) ^ int(flip), __double2loint(a));
+}
+
+// Gathers of x[r ^ m] for the R rows of a thread: from the LDS x tile, or --
+// for masks above the tile -- from the partner tile in global memory, with
+// one uniform base pointer per row block and the thread's offset shared.
+template <bool kFull, int R, typename cT>
+__device__ __forceinline__ void gather(uint64_t m, uint32_t tid, uint64_t tile_rows, uint64_t base,
+                                       const cT *xt, const cT *__restrict__ xv, cT (&xg)[R]) {
     const uint64_t mlo = m & (tile_rows - 1), mhi = m & ~(tile_rows - 1);
     const uint32_t tx = tid ^ uint32_t(mlo & (kApplyThreads - 1));
     const uint32_t mk = uint32_t(mlo >> kApplyThreadBits);
-    const cT *__restrict__ far = xv + (base ^ mhi);
-    cT xg[R];
     if (mhi) {
+        const cT *__restrict__ far = xv + (base ^ mhi);
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            if (kFull || tid + uint64_t(k) * kApplyThreads < tile_rows)
-                xg[k] = far[tx + ((uint32_t(k) ^ mk) << kApplyThreadBits)];
+            if (!kFull && tid + uint64_t(k) * kApplyThreads >= tile_rows) break;
+            const cT *__restrict__ fk = far + (size_t(uint32_t(k) ^ mk) << kApplyThreadBits);
+            xg[k] = fk[tx];
         }
     } else {
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            if (kFull || tid + uint64_t(k) * kApplyThreads < tile_rows)
-                xg[k] = xt[tx + ((uint32_t(k) ^ mk) << kApplyThreadBits)];
+            if (!kFull && tid + uint64_t(k) * kApplyThreads >= tile_rows) break;
+            xg[k] = xt[tx + ((uint32_t(k) ^ mk) << kApplyThreadBits)];
         }
     }
+}
+
+// y += D_g(r) x[r ^ m] for the R rows of a thread, x already gathered.  D_g is
+// formed from the chunk's pattern coefficients: thread-bit patterns once per
+// thread, then -- only when the group has them -- per row the row-bit table
+// entry sU[urow*R + k] and the straddling patterns.  Products go to the FP64
+// accumulators ar/ai, or (mixed) to the FP32 block partials br/bi.
+template <bool kMixed, bool kReal, bool kFull, int R, typename cT>
+__device__ __forceinline__ void accumulate(const ApplyGroup &gd, uint32_t tid, uint64_t tile_rows,
+                                           uint32_t zero_row, const double2 *sA, const uint32_t *sZ,
+                                           const double2 *sU, const cT (&xg)[R], double (&ar)[R],
+                                           double (&ai)[R], float (&br)[R], float (&bi)[R]) {
+    double cr = 0.0, ci = 0.0;
+    for (uint32_t p = gd.p0; p < gd.ps; ++p) {
+        const double2 a = sA[p];
+        const uint32_t flip = uint32_t(__popc(tid & sZ[p])) << 31;
+        cr += flip_sign(a.x, flip);
+        if (!kReal) ci += flip_sign(a.y, flip);
+    }
+    if (gd.u < 0 && gd.ps == gd.pk) {
+        // D is the same for the R rows of this thread
+        const float fr = float(cr), fi = float(ci);
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            if (!kFull && tid + uint64_t(k) * kApplyThreads >= tile_rows) break;
+            const cT xb = xg[k];
+            if (!kMixed) {
+                ar[k] = fma(cr, xb.x, ar[k]);
+                ai[k] = fma(cr, xb.y, ai[k]);
+                if (!kReal) {
+                    ar[k] = fma(-ci, xb.y, ar[k]);
+                    ai[k] = fma(ci, xb.x, ai[k]);
+                }
+            } else {
+                br[k] = fmaf(fr, xb.x, br[k]);
+                bi[k] = fmaf(fr, xb.y, bi[k]);
+                if (!kReal) {
+                    br[k] = fmaf(-fi, xb.y, br[k]);
+                    bi[k] = fmaf(fi, xb.x, bi[k]);
+                }
+            }
+        }
+        return;
+    }
+    const double2 *su = sU + (gd.u >= 0 ? uint32_t(gd.u) : zero_row) * R;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         if (!kFull && tid + uint64_t(k) * kApplyThreads >= tile_rows) break;
+        const double2 uk = su[k];
+        double dr = cr + uk.x, di = kReal ? 0.0 : ci + uk.y;
+        for (uint32_t p = gd.ps; p < gd.pk; ++p) {
+            const double2 a = sA[p];
+            const uint32_t zl = uint32_t(__builtin_amdgcn_readfirstlane(int(sZ[p])));
+            const uint32_t flip =
+                uint32_t(__popc(tid & zl) + __popc(uint32_t(k) & (zl >> kApplyThreadBits))) << 31;
+            dr += flip_sign(a.x, flip);
+            if (!kReal) di += flip_sign(a.y, flip);
+        }
         const cT xb = xg[k];
         if (!kMixed) {
-            ar[k] = fma(dr[k], xb.x, ar[k]);
-            ai[k] = fma(dr[k], xb.y, ai[k]);
+            ar[k] = fma(dr, xb.x, ar[k]);
+            ai[k] = fma(dr, xb.y, ai[k]);
             if (!kReal) {
-                ar[k] = fma(-di[k], xb.y, ar[k]);
-                ai[k] = fma(di[k], xb.x, ai[k]);
+                ar[k] = fma(-di, xb.y, ar[k]);
+                ai[k] = fma(di, xb.x, ai[k]);
             }
         } else {
-            const float fr = float(dr[k]);
+            const float fr = float(dr);
             br[k] = fmaf(fr, xb.x, br[k]);
             bi[k] = fmaf(fr, xb.y, bi[k]);
             if (!kReal) {
-                const float fi = float(di[k]);
+                const float fi = float(di);
                 br[k] = fmaf(-fi, xb.y, br[k]);
                 bi[k] = fmaf(fi, xb.x, bi[k]);
             }
@@ -197,20 +193,71 @@ __device__ __forceinline__ void apply_group(
     }
 }
 
-template <bool kMixed, bool kNarrow, bool kFull, bool kPatLDS>
+// y is written once and not read again by the kernel: nontemporal stores, so
+// they do not displace x (which the partner tiles still gather) from the caches.
+template <typename cT>
+__device__ __forceinline__ void store_y(cT *p, cT v) {
+    __builtin_nontemporal_store(v.x, &p->x);
+    __builtin_nontemporal_store(v.y, &p->y);
+}
+
+// A_p = sum_t (-1)^popcount(base & z_t) c'_t over the pattern's terms
+// [t0, t1), in term order.  The loads of four terms are issued together; the
+// sum itself stays sequential.
+template <bool kNarrow>
+__device__ __forceinline__ double2 pattern_coeff(uint64_t base, uint32_t t0, uint32_t t1,
+                                                 const uint64_t *__restrict__ ptz,
+                                                 const double2 *__restrict__ ptc) {
+    double a_r = 0.0, a_i = 0.0;
+    uint32_t t = t0;
+    for (; t + 4 <= t1; t += 4) {
+        double2 cc[4];
+        uint32_t par[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            cc[j] = ptc[t + j];
+            par[j] = parity_of<kNarrow>(base, ptz[t + j]);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (par[j]) { a_r -= cc[j].x; a_i -= cc[j].y; }
+            else { a_r += cc[j].x; a_i += cc[j].y; }
+        }
+    }
+    for (; t < t1; ++t) {
+        const double2 cc = ptc[t];
+        if (parity_of<kNarrow>(base, ptz[t])) { a_r -= cc.x; a_i -= cc.y; }
+        else { a_r += cc.x; a_i += cc.y; }
+    }
+    return make_double2(a_r, a_i);
+}
+
+// LDS layout of k_apply_tiled: x tile | sA[max_p] | sZ[max_p] | sU[max_u * R]
+template <typename cT>
+__host__ __device__ inline size_t apply_lds_offsets(uint64_t tile_rows, uint32_t max_p, uint32_t R,
+                                                    uint32_t max_u, size_t *oA, size_t *oZ,
+                                                    size_t *oU) {
+    const size_t a = (size_t(tile_rows) * sizeof(cT) + 15) & ~size_t(15);
+    const size_t z = a + size_t(max_p) * sizeof(double2);
+    const size_t u = (z + size_t(max_p) * sizeof(uint32_t) + 15) & ~size_t(15);
+    if (oA) *oA = a;
+    if (oZ) *oZ = z;
+    if (oU) *oU = u;
+    return u + size_t(max_u) * R * sizeof(double2);
+}
+
+template <bool kMixed, bool kNarrow, bool kFull>
 __global__ __launch_bounds__(kApplyThreads) void k_apply_tiled(
-    uint64_t N, uint32_t G, uint32_t P, uint32_t ntiles, uint32_t tiles_per_xcd,
-    const uint64_t *__restrict__ gx, const uint8_t *__restrict__ gkind,
-    const uint32_t *__restrict__ gpat, const uint32_t *__restrict__ gcls,
-    const int32_t *__restrict__ gU, const uint32_t *__restrict__ ug, uint32_t nU,
-    const uint32_t *__restrict__ pzl, const uint32_t *__restrict__ ptptr,
-    const uint64_t *__restrict__ ptz, const double2 *__restrict__ ptc,
-    const void *__restrict__ xin, void *__restrict__ yout, double *__restrict__ partials) {
+    uint64_t N, uint32_t ntiles, uint32_t tiles_per_xcd, uint32_t ngrp, uint32_t nchunk,
+    uint32_t max_p, uint32_t zero_row, const ApplyGroup *__restrict__ grp, const ApplyChunk *__restrict__ chunks,
+    const uint32_t *__restrict__ ug, const uint32_t *__restrict__ pzl,
+    const uint32_t *__restrict__ ptptr, const uint64_t *__restrict__ ptz,
+    const double2 *__restrict__ ptc, const void *__restrict__ xin, void *__restrict__ yout,
+    double *__restrict__ partials) {
     using Shape = ApplyShape<kMixed>;
     constexpr int L = Shape::L, R = Shape::R;
     using cT = typename std::conditional<kMixed, float2, double2>::type;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    cT *xt = reinterpret_cast<cT *>(smem_raw);
 
     const uint32_t tid = threadIdx.x;
     const uint32_t v = blockIdx.x / ntiles;
@@ -220,38 +267,14 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_tiled(
     const uint64_t base = uint64_t(tile) * tile_rows;
     const cT *__restrict__ xv = reinterpret_cast<const cT *>(xin) + uint64_t(v) * N;
 
+    size_t oA, oZ, oU;
+    apply_lds_offsets<cT>(tile_rows, max_p, R, 0, &oA, &oZ, &oU);
+    cT *xt = reinterpret_cast<cT *>(smem_raw);
+    double2 *sA = reinterpret_cast<double2 *>(smem_raw + oA);
+    uint32_t *sZ = reinterpret_cast<uint32_t *>(smem_raw + oZ);
+    double2 *sU = reinterpret_cast<double2 *>(smem_raw + oU);
+
     for (uint64_t i = tid; i < tile_rows; i += kApplyThreads) xt[i] = xv[base + i];
-    // per-tile pattern coefficients A_p = sum_t (-1)^popcount(base & z_t) c'_t
-    double2 *sA = reinterpret_cast<double2 *>(smem_raw + tile_rows * sizeof(cT));
-    if (kPatLDS) {
-        for (uint32_t p = tid; p < P; p += kApplyThreads) {
-            double a_r = 0.0, a_i = 0.0;
-            const uint32_t t1 = ptptr[p + 1];
-            for (uint32_t t = ptptr[p]; t < t1; ++t) {
-                const double2 c = ptc[t];
-                if (parity_of<kNarrow>(base, ptz[t])) { a_r -= c.x; a_i -= c.y; }
-                else { a_r += c.x; a_i += c.y; }
-            }
-            sA[p] = make_double2(a_r, a_i);
-        }
-    }
-    __syncthreads();
-    // per-tile row-bit table: sU[u*R + k] = sum over the row-bit patterns of
-    // group ug[u] of (-1)^popcount(k & (zl >> thread bits)) * A_p
-    double2 *sU = sA + P;
-    if (kPatLDS && nU) {
-        for (uint32_t i = tid; i < nU * R; i += kApplyThreads) {
-            const uint32_t u = i / R, k = i % R, gg = ug[u];
-            double s_r = 0.0, s_i = 0.0;
-            for (uint32_t p = gcls[2 * gg + 1]; p < gpat[gg + 1]; ++p) {
-                const double2 a = sA[p];
-                if (__popc(k & (pzl[p] >> kApplyThreadBits)) & 1u) { s_r -= a.x; s_i -= a.y; }
-                else { s_r += a.x; s_i += a.y; }
-            }
-            sU[i] = make_double2(s_r, s_i);
-        }
-        __syncthreads();
-    }
 
     // accumulators: FP64 (ar, ai) or FP32 running sum s, compensation c, block partial b
     double ar[R], ai[R];
@@ -263,17 +286,16 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_tiled(
     }
 
     int since_flush = 0;
-    for (uint32_t g = 0; g < G; ++g) {
-        // kind 1: every c' of the group is real, so D_g(r) is real
-        if (gkind[g] == 1)
-            apply_group<kMixed, true, kFull, kPatLDS, R>(g, tid, tile_rows, base, gx, gpat, gcls,
-                                                         gU, pzl, ptptr, ptz, ptc, sA, sU, xt, xv,
-                                                         ar, ai, br, bi);
+    // one group: accumulate, and every kCompBlock groups add the FP32 block
+    // partials into the running sums with TwoSum
+    auto step = [&](const ApplyGroup &gd, const cT (&xg)[R]) {
+        if (gd.real)
+            accumulate<kMixed, true, kFull, R>(gd, tid, tile_rows, zero_row, sA, sZ, sU, xg, ar, ai,
+                                               br, bi);
         else
-            apply_group<kMixed, false, kFull, kPatLDS, R>(g, tid, tile_rows, base, gx, gpat, gcls,
-                                                          gU, pzl, ptptr, ptz, ptc, sA, sU, xt, xv,
-                                                          ar, ai, br, bi);
-        if (kMixed && (++since_flush == kCompBlock || g + 1 == G)) {
+            accumulate<kMixed, false, kFull, R>(gd, tid, tile_rows, zero_row, sA, sZ, sU, xg, ar,
+                                                ai, br, bi);
+        if (kMixed && ++since_flush == kCompBlock) {
             since_flush = 0;
 #pragma unroll
             for (int k = 0; k < R; ++k) {
@@ -284,6 +306,50 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_tiled(
                 ci[k] += e;
                 br[k] = bi[k] = 0.0f;
             }
+        }
+    };
+    cT xg[R];
+    for (uint32_t c = 0; c < nchunk; ++c) {
+        const ApplyChunk ch = chunks[c];
+        if (c) __syncthreads();  // the previous chunk's tables are no longer read
+        // per-tile pattern coefficients A_p = sum_t (-1)^popcount(base & z_t) c'_t
+        for (uint32_t p = tid; p < ch.p1 - ch.p0; p += kApplyThreads) {
+            const uint32_t pa = ch.p0 + p;
+            sA[p] = pattern_coeff<kNarrow>(base, ptptr[pa], ptptr[pa + 1], ptz, ptc);
+            sZ[p] = pzl[pa];
+        }
+        __syncthreads();
+        // per-tile row-bit table: sU[u*R + k] = sum over the row-bit patterns of
+        // the group of (-1)^popcount(k & (zl >> thread bits)) * A_p; row
+        // zero_row stays zero for groups without such patterns
+        for (uint32_t i = tid; i < (ch.u1 - ch.u0 + 1) * R; i += kApplyThreads) {
+            const uint32_t u = i / R, k = i % R;
+            double s_r = 0.0, s_i = 0.0;
+            if (u < ch.u1 - ch.u0) {
+                const ApplyGroup &gg = grp[ug[ch.u0 + u]];
+                for (uint32_t p = gg.pk; p < gg.p1; ++p) {
+                    const double2 a = sA[p];
+                    if (__popc(k & (sZ[p] >> kApplyThreadBits)) & 1u) { s_r -= a.x; s_i -= a.y; }
+                    else { s_r += a.x; s_i += a.y; }
+                }
+            }
+            sU[(u < ch.u1 - ch.u0 ? u : zero_row) * R + k] = make_double2(s_r, s_i);
+        }
+        __syncthreads();
+        for (uint32_t g = ch.g0; g < ch.g1; ++g) {
+            const ApplyGroup gd = grp[g];
+            gather<kFull, R>(gd.m, tid, tile_rows, base, xt, xv, xg);
+            step(gd, xg);
+        }
+    }
+    if (kMixed && since_flush) {
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            float e;
+            two_sum(sr[k], br[k], sr[k], e);
+            cr[k] += e;
+            two_sum(si[k], bi[k], si[k], e);
+            ci[k] += e;
         }
     }
 
@@ -297,11 +363,11 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_tiled(
         if (!kMixed) {
             yr = ar[k];
             yi = ai[k];
-            if (yv) reinterpret_cast<double2 *>(yv)[base + l] = make_double2(yr, yi);
+            if (yv) store_y(reinterpret_cast<double2 *>(yv) + base + l, make_double2(yr, yi));
         } else {
             yr = double(sr[k]) + double(cr[k]);
             yi = double(si[k]) + double(ci[k]);
-            if (yv) reinterpret_cast<float2 *>(yv)[base + l] = make_float2(sr[k] + cr[k], si[k] + ci[k]);
+            if (yv) store_y(reinterpret_cast<float2 *>(yv) + base + l, make_float2(sr[k] + cr[k], si[k] + ci[k]));
         }
         if (partials) {
             const cT xr = xt[l];
@@ -356,22 +422,20 @@ static int launch_tiled(const qr_op *op, DeviceTables *t, uint32_t nvec, const v
     const uint64_t N = uint64_t(1) << op->nq;
     const uint64_t tile_rows = N < (uint64_t(1) << Shape::L) ? N : (uint64_t(1) << Shape::L);
     const DeviceTables::Patterns &pt = t->pat[kMixed ? 1 : 0];
-    const bool plds = pt.P + size_t(pt.nU) * Shape::R <= kMaxPatLDS;
-    const size_t shmem = size_t(tile_rows) * sizeof(cT) +
-                         (plds ? (size_t(pt.P) + size_t(pt.nU) * Shape::R) * sizeof(double2) : 0);
+    // the row-bit table holds max_u rows plus one zero row
+    const size_t shmem = apply_lds_offsets<cT>(tile_rows, pt.max_p, Shape::R, pt.max_u + 1,
+                                               nullptr, nullptr, nullptr);
+    if (shmem > kApplyMaxLDS) return fail(QR_ERR_SIZE, "apply: LDS stage too large");
     const uint32_t tpx = (ntiles % 8 == 0) ? ntiles / 8 : 0;
     const bool full = tile_rows == (uint64_t(1) << Shape::L);
-    auto kern = full ? (plds ? k_apply_tiled<kMixed, kNarrow, true, true>
-                             : k_apply_tiled<kMixed, kNarrow, true, false>)
-                     : (plds ? k_apply_tiled<kMixed, kNarrow, false, true>
-                             : k_apply_tiled<kMixed, kNarrow, false, false>);
+    auto kern = full ? k_apply_tiled<kMixed, kNarrow, true> : k_apply_tiled<kMixed, kNarrow, false>;
     QR_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem)));
     const uint64_t nblocks = uint64_t(ntiles) * nvec;
     if (nblocks > 0x7FFFFFFFull) return fail(QR_ERR_SIZE, "apply grid too large");
-    hipLaunchKernelGGL(kern, dim3(unsigned(nblocks)), dim3(kApplyThreads), shmem, s, N,
-                       uint32_t(op->G), pt.P, ntiles, tpx, t->gx, t->gkind, pt.gpat, pt.gcls, pt.gU,
-                       pt.ug, pt.nU, pt.pzl, pt.ptptr, pt.ptz, pt.ptc, d_x, d_y, partials);
+    hipLaunchKernelGGL(kern, dim3(unsigned(nblocks)), dim3(kApplyThreads), shmem, s, N, ntiles,
+                       tpx, pt.ngrp, pt.nchunk, pt.max_p, pt.max_u, pt.grp, pt.chunk, pt.ug, pt.pzl, pt.ptptr, pt.ptz,
+                       pt.ptc, d_x, d_y, partials);
     QR_HIP_CHECK(hipGetLastError());
     return QR_OK;
 }
@@ -381,7 +445,9 @@ int launch_pauli_apply(const qr_op *op, DeviceTables *t, int32_t precision, uint
     const bool mixed = precision == QR_PREC_MIXED, narrow = op->nq <= 32;
     const int Lmax = mixed ? ApplyShape<true>::L : ApplyShape<false>::L;
     const int L = int(op->nq) < Lmax ? int(op->nq) : Lmax;
-    int rc = device_patterns(op, t, mixed ? 1 : 0, L, kApplyThreadBits);
+    const uint32_t pcap = mixed ? ApplyShape<true>::kPatChunk : ApplyShape<false>::kPatChunk;
+    const uint32_t ucap = mixed ? ApplyShape<true>::kTableRows : ApplyShape<false>::kTableRows;
+    int rc = device_patterns(op, t, mixed ? 1 : 0, L, kApplyThreadBits, pcap, ucap);
     if (rc) return rc;
     const uint64_t N = uint64_t(1) << op->nq;
     const uint32_t ntiles = uint32_t(N >> L);

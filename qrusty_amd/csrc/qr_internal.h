@@ -33,6 +33,24 @@ struct RankEntry {
     int32_t sigma;
 };
 
+// One (sub)group of the matrix-free apply: x mask, its pattern range split by
+// class -- [p0, ps) z bits among the thread-index bits only, [ps, pk)
+// straddling, [pk, p1) row-index bits only -- and its row in the chunk's
+// per-tile table (-1 if it has no row-index patterns).  Pattern indices are
+// relative to the chunk.  32 bytes, read with one scalar load.
+struct ApplyGroup {
+    uint64_t m;
+    uint32_t p0, ps, pk, p1;
+    int32_t u;
+    uint32_t real;  // every coefficient of the group is real
+};
+
+// Groups [g0, g1) whose patterns [p0, p1) and table rows [u0, u1) are staged
+// in LDS together.
+struct ApplyChunk {
+    uint32_t g0, g1, p0, p1, u0, u1, pad0, pad1;
+};
+
 // Device copies of an operator's tables (one set per device that used it).
 struct DeviceTables {
     bool ready = false;
@@ -45,15 +63,13 @@ struct DeviceTables {
     uint8_t *gkind = nullptr;    // [G]   0 complex, 1 real, 2 imaginary coefficients
     double *partials = nullptr;  // workspace for <x|y> block partials
     size_t partials_cap = 0;     // in doubles
-    // matrix-free apply: terms of each group merged by their z bits inside a
-    // tile of 2^L rows ("patterns"), one table set per tile size
+    // matrix-free apply tables, one set per precision (tile size)
     struct Patterns {
         int L = -1;
-        uint32_t P = 0, nU = 0;
-        uint32_t *gpat = nullptr;   // [G+1] pattern range of each group
-        uint32_t *gcls = nullptr;   // [2G]  first straddling / first row-index pattern
-        int32_t *gU = nullptr;      // [G]   row of the group in the per-tile k table, or -1
-        uint32_t *ug = nullptr;     // [nU]  groups that have row-index patterns
+        uint32_t P = 0, ngrp = 0, nchunk = 0, max_p = 0, max_u = 0;
+        ApplyGroup *grp = nullptr;    // [ngrp]
+        ApplyChunk *chunk = nullptr;  // [nchunk]
+        uint32_t *ug = nullptr;     // [U]   group (index into grp) of each table row
         uint32_t *pzl = nullptr;    // [P]   z & (2^L - 1) of the pattern
         uint32_t *ptptr = nullptr;  // [P+1] term range of each pattern
         uint64_t *ptz = nullptr;    // [T]   z masks in pattern order
@@ -61,17 +77,22 @@ struct DeviceTables {
     } pat[2];
 };
 
-// Host build of the pattern tables for tile size L and tb thread-index bits
-// (see qr_apply.hip).  Patterns of a group are ordered by class: z bits only
-// among the thread-index bits, then straddling, then only row-index bits.
+// Host build of the apply tables for tile size L, tb thread-index bits, at
+// most pcap patterns and ucap table rows per chunk (see qr_apply.hip).  A
+// group with more than pcap patterns is split into several ApplyGroups with
+// the same mask.
 struct HostPatterns {
-    std::vector<uint32_t> gpat, gcls, ug, pzl, ptptr;
-    std::vector<int32_t> gU;
+    std::vector<ApplyGroup> grp;
+    std::vector<ApplyChunk> chunk;
+    std::vector<uint32_t> ug, pzl, ptptr;
     std::vector<uint64_t> ptz;
     std::vector<double> ptc;  // 2 per term
+    uint32_t max_p = 0, max_u = 0;
 };
-void build_patterns(const qr_op *op, int L, int tb, HostPatterns &out);
-int device_patterns(const qr_op *op, DeviceTables *t, int slot, int L, int tb);
+void build_patterns(const qr_op *op, int L, int tb, uint32_t pcap, uint32_t ucap,
+                    HostPatterns &out);
+int device_patterns(const qr_op *op, DeviceTables *t, int slot, int L, int tb, uint32_t pcap,
+                    uint32_t ucap);
 
 }  // namespace qr
 
