@@ -450,6 +450,8 @@ int launch_pauli_apply(const qr_op *op, DeviceTables *t, int32_t precision, uint
     int rc = device_patterns(op, t, mixed ? 1 : 0, L, kApplyThreadBits, pcap, ucap);
     if (rc) return rc;
     const uint64_t N = uint64_t(1) << op->nq;
+    if ((N >> L) * uint64_t(nvec) > 0x7FFFFFFFull || nvec == 0)
+        return fail(QR_ERR_SIZE, "apply: state too large for one launch (or nvec == 0)");
     const uint32_t ntiles = uint32_t(N >> L);
     double *partials = nullptr;
     if (d_expect) {
