@@ -280,7 +280,7 @@ def bench_batch(args, torch, qa):
     m = statistics.mean(ms)
     # algorithmic bytes: CSR written (24 B/nnz + indptr), then y = A x (CSR
     # re-read by the single-operator route only; the fused route reads x)
-    big = [k for k, h in enumerate(b._h) if h.G > 4096]
+    big = [k for k, h in enumerate(b._h) if h.G > 6144]
     nnz_big = sum(b._h[k].nnz for k in big)
     byts = 24 * nnz + 8 * (rows + len(ops)) + 24 * nnz_big + 32 * rows
     res = {"workload": "batch of 256 uneven random operators (8..18 qubits, 10..5000 terms): "

@@ -173,7 +173,7 @@ int qr_pauli_apply(const qr_op *op, int32_t precision, uint32_t nvec, const void
  * y_k is computed from the freshly built entries (FP64 FMAs).
  *   d_indptr[k], d_indices[k], d_data[k] : CSR outputs sized as for
  *       qr_csr_build, or NULL (all three, or the arrays themselves) to keep
- *       only y_k -- except for operators with more than 4096 distinct x masks,
+ *       only y_k -- except for operators with more than 6144 distinct x masks,
  *       which run through the single-operator kernels and need them
  *   d_x[k], d_y[k] : complex128 states of 2^n_k entries (n_k <= 32)
  *   tuning : NULL = the tuned (or default) parameters

@@ -441,7 +441,7 @@ class Batch:
 
     Output buffers are allocated once and reused by every run(), so the
     recorded graph is replayed; CSR outputs are kept only when keep_csr
-    (operators with more than 4096 distinct x masks always keep theirs).
+    (operators with more than 6144 distinct x masks always keep theirs).
     """
 
     def __init__(self, ops, keep_csr: bool = True):
@@ -462,7 +462,7 @@ class Batch:
             N = 1 << h.num_qubits
             self.x.append(torch.zeros(N, dtype=torch.complex128, device=dev))
             self.y.append(torch.zeros(N, dtype=torch.complex128, device=dev))
-            if keep_csr or h.G > 4096:
+            if keep_csr or h.G > 6144:
                 self._csr.append((torch.empty(N + 1, dtype=torch.int64, device=dev),
                                   torch.empty(h.nnz, dtype=torch.int64, device=dev),
                                   torch.empty(h.nnz, dtype=torch.complex128, device=dev)))

@@ -67,7 +67,7 @@ class QrError(Exception):
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(
-            f"qrusty_amd: {LIB_PATH} is not built; run `python -m qrusty_amd.build` "
+            f"qrusty_amd: {LIB_PATH} is not built; run `python qrusty_amd/build.py` "
             "(the HIP kernels have no CPU fallback)")
     try:
         L = ctypes.CDLL(LIB_PATH)

@@ -1,6 +1,7 @@
 """Builds libqrhip.so (HIP kernels + C ABI) for gfx950 with hipcc, in-tree.
 
-Usage: python -m qrusty_amd.build   (or __graft_entry__.build())
+Usage: python qrusty_amd/build.py   (or __graft_entry__.build()); it does not
+import the package, which would load the library being rebuilt.
 """
 from __future__ import annotations
 

@@ -9,8 +9,8 @@
 // bit-identical to the single-operator build -- and, fused, the same rows of
 // y = A x from the entries still in registers.  No item depends on another,
 // so every block leaves the loop as soon as the counter passes the end.
-// Operators whose G exceeds the LDS stage (kStageMaxG) are built and applied
-// by the single-operator kernels, recorded in the same graph.
+// Operators whose G exceeds the largest LDS stage (kStageMaxGLean) are built
+// and applied by the single-operator kernels, recorded in the same graph.
 //
 // Tuning knobs (qr_tuning): tiles per item (work granularity), resident
 // blocks per CU (grid size), and the item order (input order or largest
@@ -90,10 +90,13 @@ struct qr_batch {
 
 namespace {
 
-// register classes: group chunks per thread in {1, 2, kStageMaxChunks}.  A
-// batch operator builds all 2^n rows, so n <= 32 and masks are 32-bit.
-constexpr int kClasses = 3;
-inline int class_of(uint32_t nch) { return nch <= 1 ? 0 : nch <= 2 ? 1 : 2; }
+// register classes: group chunks per thread in {1, 2, kStageMaxChunks,
+// kStageLeanChunks}.  A batch operator builds all 2^n rows, so n <= 32 and
+// masks are 32-bit.
+constexpr int kClasses = 4;
+inline int class_of(uint32_t nch) {
+    return nch <= 1 ? 0 : nch <= 2 ? 1 : nch <= uint32_t(kStageMaxChunks) ? 2 : 3;
+}
 
 struct Plan {
     std::vector<BatchOpDesc> desc;            // grouped by class, in item order
@@ -124,9 +127,9 @@ int make_plan(qr_batch *b, uint64_t *const *d_indptr, uint64_t *const *d_indices
         int rc = device_tables(op, &t);
         if (rc) return rc;
         const uint32_t G = uint32_t(op->G);
-        if (op->G > kStageMaxG) {
+        if (op->G > kStageMaxGLean) {
             if (!has_csr || !d_indptr || !d_indptr[k])
-                return fail(QR_ERR_ARG, "batch operators with more than 4096 distinct x masks "
+                return fail(QR_ERR_ARG, "batch operators with more than 6144 distinct x masks "
                                         "need CSR output buffers");
             plan.big.push_back(k);
             continue;
@@ -185,6 +188,7 @@ int record(qr_batch *b, const Plan &plan, const qr_tuning &tu, uint64_t *const *
     launch_class<true, 1>(b, plan, 0, cap, tpi, s);
     launch_class<true, 2>(b, plan, 1, cap, tpi, s);
     launch_class<true, kStageMaxChunks>(b, plan, 2, cap, tpi, s);
+    launch_class<true, kStageLeanChunks>(b, plan, 3, cap, tpi, s);
     QR_HIP_CHECK(hipGetLastError());
     for (uint32_t k : plan.big) {
         const qr_op *op = b->ops[k];
@@ -212,11 +216,12 @@ int prepare(qr_batch *b, const Plan &plan) {
         hipDeviceProp_t prop;
         QR_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
         b->ncu = prop.multiProcessorCount;
-        const int maxsh = int(stage_bytes(kStageMaxG, 10, nullptr));
+        const int maxsh = int(stage_bytes(kStageMaxGLean, 10, nullptr));
         const void *kerns[kClasses] = {
             reinterpret_cast<const void *>(k_batch<true, 1>),
             reinterpret_cast<const void *>(k_batch<true, 2>),
-            reinterpret_cast<const void *>(k_batch<true, kStageMaxChunks>)};
+            reinterpret_cast<const void *>(k_batch<true, kStageMaxChunks>),
+            reinterpret_cast<const void *>(k_batch<true, kStageLeanChunks>)};
         for (const void *kf : kerns)
             QR_HIP_CHECK(hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, maxsh));
     }

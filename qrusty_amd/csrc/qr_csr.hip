@@ -139,7 +139,7 @@ __global__ __launch_bounds__(kBuildThreads) void k_csr_build(
 #endif
 }
 
-// LDS-staged build (used when G <= kStageMaxG): one block per tile of the
+// LDS-staged build (used when G <= kStageMaxGLean): one block per tile of the
 // shared tile builder (qr_tile.h), without the fused matvec.
 template <bool kNarrow, int kChunks>
 __global__ __launch_bounds__(kStageThreads) void k_csr_build_staged(TileOp op) {
@@ -166,7 +166,7 @@ int launch_csr_build(const qr_op *op, const DeviceTables *t, uint64_t row_begin,
     }
     if (nrows == 0) return QR_OK;
     const bool narrow = op->nq <= 32;
-    if (G <= kStageMaxG) {
+    if (G <= kStageMaxGLean) {
         const uint32_t jw = stage_jw_log2(G);
         const uint32_t nch = (G + (1u << jw) - 1) >> jw;
         const uint32_t tl2 = 10 - jw + 4;
@@ -178,11 +178,13 @@ int launch_csr_build(const qr_op *op, const DeviceTables *t, uint64_t row_begin,
                    reinterpret_cast<double2 *>(d_data), nullptr, nullptr, nullptr,
                    row_begin, row_end, G, jw, nch, dbuf};
         auto kst = narrow ? (nch <= 1 ? k_csr_build_staged<true, 1>
-                                      : nch <= 2 ? k_csr_build_staged<true, 2>
-                                                 : k_csr_build_staged<true, kStageMaxChunks>)
+                             : nch <= 2 ? k_csr_build_staged<true, 2>
+                             : nch <= uint32_t(kStageMaxChunks) ? k_csr_build_staged<true, kStageMaxChunks>
+                                                                : k_csr_build_staged<true, kStageLeanChunks>)
                           : (nch <= 1 ? k_csr_build_staged<false, 1>
-                                      : nch <= 2 ? k_csr_build_staged<false, 2>
-                                                 : k_csr_build_staged<false, kStageMaxChunks>);
+                             : nch <= 2 ? k_csr_build_staged<false, 2>
+                             : nch <= uint32_t(kStageMaxChunks) ? k_csr_build_staged<false, kStageMaxChunks>
+                                                                : k_csr_build_staged<false, kStageLeanChunks>);
         QR_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(kst),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem)));
         hipLaunchKernelGGL(kst, dim3(unsigned(nt)), dim3(kStageThreads), shmem, s, top);

@@ -32,6 +32,11 @@ constexpr int kBuildM = 16;            // batches per tile (4 varying row bits)
 constexpr int kStageThreads = 1024;
 constexpr uint32_t kStageMaxG = 4096;  // 24 B per entry: 96 KB of LDS for one row
 constexpr int kStageMaxChunks = 4;
+// "lean" class: up to 6 chunks (G <= 6144, a 144 KB row stage); only the rank
+// data and term offsets stay in registers, the first term of each group is
+// re-read (from L2) for every batch
+constexpr int kStageLeanChunks = 6;
+constexpr uint32_t kStageMaxGLean = 6144;
 
 template <bool kNarrow>
 __device__ __forceinline__ uint32_t parity_and(uint64_t a, uint64_t b) {
@@ -63,11 +68,18 @@ struct TileOp {
 };
 
 // LDS bytes for one operator's stage: (one or two) x (index + value) x batch,
-// plus 2 x 16 wave partials for the fused matvec.
+// plus 2 x 16 wave partials for the fused matvec.  The stage can be
+// double-buffered (the copy-out of batch m overlaps the build of m+1) when both
+// buffers fit in QR_STAGE_DBUF_MAX bytes; by default it is not: a single stage
+// (48 KB at G = 2000) lets two 1024-thread blocks share a CU, which overlaps
+// one block's copy-out with the other's build and measured 20% faster.
+#ifndef QR_STAGE_DBUF_MAX
+#define QR_STAGE_DBUF_MAX 0
+#endif
 __host__ __device__ inline size_t stage_bytes(uint32_t G, uint32_t jw_log2, uint32_t *dbuf_out) {
     const size_t batch_pad = ((size_t(G) << (10 - jw_log2)) + 1) & ~size_t(1);
     const size_t one = batch_pad * (sizeof(uint64_t) + sizeof(double2));
-    const uint32_t dbuf = 2 * one <= 96 * 1024 ? 1u : 0u;
+    const uint32_t dbuf = 2 * one <= QR_STAGE_DBUF_MAX ? 1u : 0u;
     if (dbuf_out) *dbuf_out = dbuf;
     return (dbuf ? 2 * one : one) + 2 * 16 * sizeof(double2);
 }
@@ -81,8 +93,10 @@ __host__ __device__ inline uint32_t stage_jw_log2(uint32_t G) {
 template <bool kNarrow, int kChunks, bool kMatvec>
 __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t tile,
                                                   unsigned char *smem_raw) {
-    // the second term of each group stays in registers when few chunks leave room
+    // the second term of each group stays in registers when few chunks leave
+    // room; with more than kStageMaxChunks not even the first one does
     constexpr bool kHold2 = kChunks <= 2;
+    constexpr bool kLean = kChunks > kStageMaxChunks;
     const uint32_t G = op.G, jw_log2 = op.jw_log2, nchunks = op.nchunks;
     const uint64_t row_begin = op.row_begin, row_end = op.row_end;
     const uint32_t rs_log2 = 10u - jw_log2;
@@ -116,7 +130,9 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
 
     // masks of <= 32-qubit operators are held in 32 bits
     using mask_t = typename std::conditional<kNarrow, uint32_t, uint64_t>::type;
-    int32_t rfx[kChunks], s0[kChunks], s1[kChunks], s2[kChunks], s3[kChunks];
+    // rank offsets of the four batch bits, two 16-bit values per register
+    // (|sigma| < G <= kStageMaxGLean)
+    int32_t rfx[kChunks], s01[kChunks], s23[kChunks];
     mask_t xj[kChunks], z0[kChunks], z1[kChunks];
     double2 c0[kChunks], c1[kChunks];
     uint32_t t0[kChunks], t1[kChunks];
@@ -124,7 +140,8 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
     for (int c = 0; c < kChunks; ++c) {
         const uint32_t j = (uint32_t(c) << jw_log2) + lane;
         rfx[c] = int32_t(j);
-        s0[c] = s1[c] = s2[c] = s3[c] = 0;
+        int32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+        s01[c] = s23[c] = 0;
         xj[c] = z0[c] = z1[c] = 0;
         c0[c] = c1[c] = make_double2(0.0, 0.0);
         t0[c] = t1[c] = 0;
@@ -134,17 +151,21 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
             const RankEntry en = op.rl[e];
             const uint32_t b = uint32_t(en.bit);
             const uint32_t q = b - rs_log2;  // wraps for b < rs_log2
-            if (q == 0) s0[c] = en.sigma;
-            else if (q == 1) s1[c] = en.sigma;
-            else if (q == 2) s2[c] = en.sigma;
-            else if (q == 3) s3[c] = en.sigma;
+            if (q == 0) s0 = en.sigma;
+            else if (q == 1) s1 = en.sigma;
+            else if (q == 2) s2 = en.sigma;
+            else if (q == 3) s3 = en.sigma;
             else if ((rfix >> b) & 1) rfx[c] += en.sigma;
         }
+        s01[c] = int32_t((uint32_t(s1) << 16) | (uint32_t(s0) & 0xffffu));
+        s23[c] = int32_t((uint32_t(s3) << 16) | (uint32_t(s2) & 0xffffu));
         xj[c] = mask_t(op.gx[j]);
         t0[c] = op.gterm[j];
         t1[c] = op.gterm[j + 1];
-        z0[c] = mask_t(op.tz[t0[c]]);
-        c0[c] = op.tc[t0[c]];
+        if (!kLean) {
+            z0[c] = mask_t(op.tz[t0[c]]);
+            c0[c] = op.tc[t0[c]];
+        }
         if (kHold2 && t1[c] > t0[c] + 1) {
             z1[c] = mask_t(op.tz[t0[c] + 1]);
             c1[c] = op.tc[t0[c] + 1];
@@ -162,17 +183,24 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
         for (int c = 0; c < kChunks; ++c) {
             const uint32_t j = (uint32_t(c) << jw_log2) + lane;
             if (uint32_t(c) >= nchunks || j >= G || !row_ok) continue;
-            const int32_t rank = rfx[c] + ((m & 1) ? s0[c] : 0) + ((m & 2) ? s1[c] : 0) +
-                                 ((m & 4) ? s2[c] : 0) + ((m & 8) ? s3[c] : 0);
+            const int32_t rank = rfx[c] + ((m & 1) ? int32_t(int16_t(s01[c])) : 0) +
+                                 ((m & 2) ? (s01[c] >> 16) : 0) +
+                                 ((m & 4) ? int32_t(int16_t(s23[c])) : 0) +
+                                 ((m & 8) ? (s23[c] >> 16) : 0);
+            // term index kept opaque here, so that the compiler forms the table
+            // addresses in the loop instead of holding them (64-bit each) across it
+            uint32_t tf = t0[c];
+            asm volatile("" : "+v"(tf));
             // term-order fold: first value, then + (accel.rs:191-208)
-            double2 v = signed_coeff(c0[c], parity_and<kNarrow>(r, z0[c]));
-            if (t1[c] > t0[c] + 1) {
+            double2 v = kLean ? signed_coeff(op.tc[tf], parity_and<kNarrow>(r, op.tz[tf]))
+                              : signed_coeff(c0[c], parity_and<kNarrow>(r, z0[c]));
+            if (t1[c] > tf + 1) {
                 const double2 e = kHold2 ? signed_coeff(c1[c], parity_and<kNarrow>(r, z1[c]))
-                                         : signed_coeff(op.tc[t0[c] + 1],
-                                                        parity_and<kNarrow>(r, op.tz[t0[c] + 1]));
+                                         : signed_coeff(op.tc[tf + 1],
+                                                        parity_and<kNarrow>(r, op.tz[tf + 1]));
                 v.x = v.x + e.x;
                 v.y = v.y + e.y;
-                for (uint32_t t = t0[c] + 2; t < t1[c]; ++t) {
+                for (uint32_t t = tf + 2; t < t1[c]; ++t) {
                     const double2 ct = signed_coeff(op.tc[t], parity_and<kNarrow>(r, op.tz[t]));
                     v.x = v.x + ct.x;
                     v.y = v.y + ct.y;

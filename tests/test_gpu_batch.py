@@ -4,8 +4,8 @@ Every operator of a batch gets its CSR bit-identical to the oracle's rowwise
 build (all rows for small operators, sampled rows otherwise) and y_k = A_k x_k
 within 1e-12 relative (normwise, and componentwise against
 sum_t |c'_t||x[r ^ x_t]|).  The batch mixes every register class of the
-persistent kernel (1, 2 and 4 group chunks per thread) with an operator of
-more than 4096 distinct x masks, which runs through the single-operator
+persistent kernel (1, 2, 4 and 6 group chunks per thread) with an operator of
+more than 6144 distinct x masks, which runs through the single-operator
 kernels in the same graph.  Graph replays with new states, every tuning
 candidate and the y-only mode must give bit-identical results.
 """
@@ -20,7 +20,7 @@ torch = pytest.importorskip("torch")
 
 # (qubits, terms, seed): G = distinct x masks decides the kernel class
 SPECS = [(8, 10, 1), (13, 2600, 2), (1, 2, 3), (12, 1500, 4), (6, 40, 5),
-         (13, 7000, 6), (11, 300, 7), (14, 4200, 8), (2, 3, 9), (10, 1, 10)]
+         (13, 7000, 6), (11, 300, 7), (14, 4200, 8), (2, 3, 9), (10, 1, 10), (14, 9000, 11)]
 FULL_LIMIT = 1 << 22   # entries; above this the CSR is checked on sampled rows
 
 
@@ -55,7 +55,8 @@ def batch_case():
     print(f"[gpu] batch specs {SPECS} G={Gs}", flush=True)
     # every class of the persistent kernel, and the single-operator route
     assert any(G <= 1024 for G in Gs) and any(1024 < G <= 2048 for G in Gs)
-    assert any(2048 < G <= 4096 for G in Gs) and any(G > 4096 for G in Gs)
+    assert any(2048 < G <= 4096 for G in Gs) and any(4096 < G <= 6144 for G in Gs)
+    assert any(G > 6144 for G in Gs)
     return ops, params
 
 
@@ -158,7 +159,7 @@ def test_batch_y_only(batch_case):
     yonly.run()
     torch.cuda.synchronize()
     for k, p in enumerate(params):
-        if len(np.unique(p.x)) <= 4096:
+        if len(np.unique(p.x)) <= 6144:
             assert yonly.csr(k) is None
         assert np.array_equal(yonly.y[k].cpu().numpy(), full.y[k].cpu().numpy())
 

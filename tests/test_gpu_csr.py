@@ -121,12 +121,15 @@ def test_kron_modes_match_dense():
 
 
 def test_build_rows_slices_and_wide_masks():
-    """qr_csr_build_rows on row slices, including 40-qubit operators (64-bit masks)."""
+    """qr_csr_build_rows on row slices, including 33- to 40-qubit operators (64-bit
+    masks), for every kernel class: 1, 2, 4 and 6 group chunks per thread
+    (G <= 1024, 2048, 4096, 6144) and the unstaged kernel (G > 6144)."""
     import ctypes
     import qrusty_amd as qa
     from qrusty_amd import _lib as L
     rng = np.random.default_rng(21)
-    for n, T in [(10, 50), (33, 20), (40, 64)]:
+    for n, T in [(10, 50), (33, 20), (40, 64), (12, 1500), (13, 2600), (13, 7000), (14, 9000),
+                 (36, 5000), (36, 7000)]:
         labels = _random_labels(rng, n, T, prefixes=False)
         coeffs = rng.standard_normal(T) + 1j * rng.standard_normal(T)
         print(f"[gpu] build_rows n={n} T={T}", flush=True)
@@ -259,3 +262,4 @@ def test_device_tensor_matvec_stays_on_device(hamiltonians):
     assert y.is_cuda and y.shape == (256,)
     ref = qa.spmat_dot_densevec(m, x.cpu().numpy())
     assert np.array_equal(y.cpu().numpy(), ref)
+
