@@ -185,7 +185,9 @@ typedef struct {
     uint32_t tiles_per_item;  /* row tiles per work item (1..4096)          */
     uint32_t blocks_per_cu;   /* persistent blocks per compute unit (1..8)  */
     uint32_t order;           /* 0: input order, 1: largest operator first  */
-    uint32_t reserved;
+    uint32_t split;           /* 0: matvec fused into the build (reads no CSR
+                                 back); 1: build pass, then a matvec pass over
+                                 the written CSR (needs every CSR buffer)     */
 } qr_tuning;
 
 int qr_batch_create(const qr_op *const *ops, uint32_t nops, qr_batch **out);

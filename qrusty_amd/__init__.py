@@ -487,7 +487,7 @@ class Batch:
         tu = None
         if tuning is not None:
             tu = L.QrTuning(int(tuning["tiles_per_item"]), int(tuning["blocks_per_cu"]),
-                            int(tuning["order"]), 0)
+                            int(tuning["order"]), int(tuning.get("split", 0)))
         L.check(L.lib.qr_batch_run(self._b, self._pp, self._pi, self._pd, self._px, self._py,
                                    ctypes.byref(tu) if tu is not None else None,
                                    _stream_handle(torch)))
@@ -499,7 +499,7 @@ class Batch:
         L.check(L.lib.qr_batch_tune(self._b, self._pp, self._pi, self._pd, self._px, self._py,
                                     ctypes.byref(tu), _stream_handle(torch)))
         return {"tiles_per_item": tu.tiles_per_item, "blocks_per_cu": tu.blocks_per_cu,
-                "order": tu.order}
+                "order": tu.order, "split": tu.split}
 
     def csr(self, k: int):
         """(indptr, indices, data) device tensors of operator k, or None."""

@@ -52,7 +52,7 @@ SIGNATURES = {
 class QrTuning(ctypes.Structure):
     """qr_tuning of include/qrhip.h."""
     _fields_ = [("tiles_per_item", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32),
-                ("order", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("order", ctypes.c_uint32), ("split", ctypes.c_uint32)]
 
 
 class QrError(Exception):

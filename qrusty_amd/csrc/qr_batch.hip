@@ -34,8 +34,14 @@ struct BatchOpDesc {
 
 // One persistent kernel per register class (mask width x chunks per thread):
 // items of that class only, taken from the class's own counter.
-template <bool kNarrow, int kChunks>
-__global__ __launch_bounds__(kStageThreads) void k_batch(const BatchOpDesc *__restrict__ ops,
+// Minimum waves per SIMD requested from the compiler: 8 means two resident
+// 1024-thread blocks per CU (at most 64 VGPRs), as the single-operator build
+// runs; 1 leaves the register budget alone.
+#ifndef QR_BATCH_WAVES
+#define QR_BATCH_WAVES(matvec, chunks) ((!(matvec) && (chunks) <= 2) ? 8 : 1)
+#endif
+template <bool kNarrow, int kChunks, bool kMatvec>
+__global__ __launch_bounds__(kStageThreads, QR_BATCH_WAVES(kMatvec, kChunks)) void k_batch(const BatchOpDesc *__restrict__ ops,
                                                          uint32_t nops, uint64_t total_items,
                                                          uint32_t tiles_per_item,
                                                          unsigned long long *__restrict__ counter) {
@@ -58,8 +64,87 @@ __global__ __launch_bounds__(kStageThreads) void k_batch(const BatchOpDesc *__re
         const uint64_t t0 = (item - d.item_begin) * tiles_per_item;
         const uint64_t t1 = t0 + tiles_per_item < d.tiles ? t0 + tiles_per_item : d.tiles;
         for (uint64_t t = t0; t < t1; ++t) {
-            build_tile_staged<kNarrow, kChunks, true>(d.op, t, smem_raw);
+            build_tile_staged<kNarrow, kChunks, kMatvec>(d.op, t, smem_raw);
             __syncthreads();  // the stage is reused by the next tile
+        }
+    }
+}
+
+// Split form, second pass: y_k = A_k x_k from the CSR just written, over
+// (operator, block of kMvRows rows) items taken from one counter.  Eight lanes
+// share a row and a 256-thread block covers 32 consecutive rows per step, so
+// the gathers x[r ^ x_g] of neighbouring rows share cache lines (as in
+// k_csr_matvec).  Every row of a batch operator holds exactly G entries.
+struct BatchMvDesc {
+    const uint64_t *indices;
+    const double2 *data;
+    const double2 *x;
+    double2 *y;
+    uint64_t rows, G, item_begin;
+};
+constexpr uint32_t kMvRows = 1024;
+
+__global__ __launch_bounds__(256) void k_batch_mv(const BatchMvDesc *__restrict__ ops, uint32_t nops,
+                                                  uint64_t total_items,
+                                                  unsigned long long *__restrict__ counter) {
+    __shared__ unsigned long long item_s;
+    const uint32_t lane = threadIdx.x & 7u;
+    for (;;) {
+        if (threadIdx.x == 0) item_s = atomicAdd(counter, 1ull);
+        __syncthreads();
+        const uint64_t item = item_s;
+        __syncthreads();
+        if (item >= total_items) break;
+        uint32_t lo = 0, hi = nops;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (ops[mid].item_begin <= item) lo = mid;
+            else hi = mid;
+        }
+        const BatchMvDesc &d = ops[lo];
+        const uint64_t r0 = (item - d.item_begin) * kMvRows;
+        const uint64_t r1 = r0 + kMvRows < d.rows ? r0 + kMvRows : d.rows;
+        for (uint64_t rb = r0; rb < r1; rb += 32) {
+            const uint64_t row = rb + (threadIdx.x >> 3);
+            double2 acc = make_double2(0.0, 0.0);
+            if (row < r1) {
+                const uint64_t k1 = (row + 1) * d.G;
+                uint64_t k = row * d.G + lane;
+                for (; k + 56 < k1; k += 64) {
+                    uint64_t c[8];
+                    double2 a[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        c[u] = __builtin_nontemporal_load(d.indices + k + 8 * u);
+                        const dvec2 av = __builtin_nontemporal_load(
+                            reinterpret_cast<const dvec2 *>(d.data + k + 8 * u));
+                        a[u] = make_double2(av.x, av.y);
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const double2 xb = d.x[c[u]];
+                        acc.x = fma(a[u].x, xb.x, acc.x);
+                        acc.x = fma(-a[u].y, xb.y, acc.x);
+                        acc.y = fma(a[u].x, xb.y, acc.y);
+                        acc.y = fma(a[u].y, xb.x, acc.y);
+                    }
+                }
+                for (; k < k1; k += 8) {
+                    const uint64_t c = __builtin_nontemporal_load(d.indices + k);
+                    const dvec2 av = __builtin_nontemporal_load(reinterpret_cast<const dvec2 *>(d.data + k));
+                    const double2 xb = d.x[c];
+                    acc.x = fma(av.x, xb.x, acc.x);
+                    acc.x = fma(-av.y, xb.y, acc.x);
+                    acc.y = fma(av.x, xb.y, acc.y);
+                    acc.y = fma(av.y, xb.x, acc.y);
+                }
+            }
+#pragma unroll
+            for (int off = 4; off > 0; off /= 2) {
+                acc.x += __shfl_xor(acc.x, off, 8);
+                acc.y += __shfl_xor(acc.y, off, 8);
+            }
+            if (row < r1 && lane == 0) d.y[row] = acc;
         }
     }
 }
@@ -77,6 +162,8 @@ struct qr_batch {
     hipStream_t cap = nullptr;  // private capture stream (default priority)
     BatchOpDesc *d_desc = nullptr;
     size_t d_desc_cap = 0;
+    BatchMvDesc *d_mv = nullptr;
+    size_t d_mv_cap = 0;
     unsigned long long *d_counter = nullptr;
     int ncu = 0;
     // graph of the last (buffers, tuning)
@@ -99,6 +186,9 @@ inline int class_of(uint32_t nch) {
 }
 
 struct Plan {
+    bool split = false;                       // build pass, then k_batch_mv
+    std::vector<BatchMvDesc> mv;              // split form: one per staged operator
+    uint64_t mv_items = 0;
     std::vector<BatchOpDesc> desc;            // grouped by class, in item order
     uint32_t cls_begin[kClasses + 1] = {};    // descriptor range of each class
     uint64_t cls_items[kClasses] = {};
@@ -117,6 +207,11 @@ int make_plan(qr_batch *b, uint64_t *const *d_indptr, uint64_t *const *d_indices
             return (b->ops[a]->G << b->ops[a]->nq) > (b->ops[c]->G << b->ops[c]->nq);
         });
     const uint32_t tpi = tu.tiles_per_item ? tu.tiles_per_item : 1;
+    // the split form needs every staged operator's CSR
+    plan.split = tu.split != 0;
+    for (uint32_t k = 0; plan.split && k < nops; ++k)
+        if (!(d_indices && d_indices[k] && d_data && d_data[k]))
+            return fail(QR_ERR_ARG, "the split batch form needs CSR output buffers for every operator");
     std::vector<BatchOpDesc> per[kClasses];
     for (uint32_t k : order) {
         const qr_op *op = b->ops[k];
@@ -154,6 +249,13 @@ int make_plan(qr_batch *b, uint64_t *const *d_indptr, uint64_t *const *d_indices
         plan.cls_items[c] += (d.tiles + tpi - 1) / tpi;
         plan.cls_shmem[c] = std::max(plan.cls_shmem[c], sb);
         per[c].push_back(d);
+        if (plan.split) {
+            BatchMvDesc m{d_indices[k], reinterpret_cast<const double2 *>(d_data[k]),
+                          reinterpret_cast<const double2 *>(d_x[k]),
+                          reinterpret_cast<double2 *>(d_y[k]), N, G, plan.mv_items};
+            plan.mv_items += (N + kMvRows - 1) / kMvRows;
+            plan.mv.push_back(m);
+        }
     }
     for (int c = 0; c < kClasses; ++c) {
         plan.cls_begin[c] = uint32_t(plan.desc.size());
@@ -164,19 +266,22 @@ int make_plan(qr_batch *b, uint64_t *const *d_indptr, uint64_t *const *d_indices
 }
 
 template <bool kNarrow, int kChunks>
-static void launch_class(qr_batch *b, const Plan &plan, int c, uint32_t grid_cap,
-                         uint32_t tpi, hipStream_t s);
-
-template <bool kNarrow, int kChunks>
 static void launch_class(qr_batch *b, const Plan &plan, int c, uint32_t grid_cap, uint32_t tpi,
                          hipStream_t s) {
     const uint64_t items = plan.cls_items[c];
     if (items == 0) return;
     const uint64_t grid = std::min<uint64_t>(grid_cap, items);
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, s, b->d_counter + c);
-    hipLaunchKernelGGL((k_batch<kNarrow, kChunks>), dim3(unsigned(grid)), dim3(kStageThreads),
-                       plan.cls_shmem[c], s, b->d_desc + plan.cls_begin[c],
-                       plan.cls_begin[c + 1] - plan.cls_begin[c], items, tpi, b->d_counter + c);
+    if (plan.split)
+        hipLaunchKernelGGL((k_batch<kNarrow, kChunks, false>), dim3(unsigned(grid)),
+                           dim3(kStageThreads), plan.cls_shmem[c], s, b->d_desc + plan.cls_begin[c],
+                           plan.cls_begin[c + 1] - plan.cls_begin[c], items, tpi,
+                           b->d_counter + c);
+    else
+        hipLaunchKernelGGL((k_batch<kNarrow, kChunks, true>), dim3(unsigned(grid)),
+                           dim3(kStageThreads), plan.cls_shmem[c], s, b->d_desc + plan.cls_begin[c],
+                           plan.cls_begin[c + 1] - plan.cls_begin[c], items, tpi,
+                           b->d_counter + c);
 }
 
 int record(qr_batch *b, const Plan &plan, const qr_tuning &tu, uint64_t *const *d_indptr,
@@ -190,6 +295,14 @@ int record(qr_batch *b, const Plan &plan, const qr_tuning &tu, uint64_t *const *
     launch_class<true, kStageMaxChunks>(b, plan, 2, cap, tpi, s);
     launch_class<true, kStageLeanChunks>(b, plan, 3, cap, tpi, s);
     QR_HIP_CHECK(hipGetLastError());
+    if (plan.split && plan.mv_items) {
+        // 8 resident 256-thread blocks per CU and slot
+        const uint64_t grid = std::min<uint64_t>(uint64_t(b->ncu) * 8 * bpc, plan.mv_items);
+        hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, s, b->d_counter + kClasses);
+        hipLaunchKernelGGL(k_batch_mv, dim3(unsigned(grid)), dim3(256), 0, s, b->d_mv,
+                           uint32_t(plan.mv.size()), plan.mv_items, b->d_counter + kClasses);
+        QR_HIP_CHECK(hipGetLastError());
+    }
     for (uint32_t k : plan.big) {
         const qr_op *op = b->ops[k];
         DeviceTables *t;
@@ -212,16 +325,20 @@ int prepare(qr_batch *b, const Plan &plan) {
         if (b->dev >= 0) return fail(QR_ERR_ARG, "a batch is bound to the device it first ran on");
         b->dev = dev;
         QR_HIP_CHECK(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
-        QR_HIP_CHECK(hipMalloc(&b->d_counter, kClasses * sizeof(unsigned long long)));
+        QR_HIP_CHECK(hipMalloc(&b->d_counter, (kClasses + 1) * sizeof(unsigned long long)));
         hipDeviceProp_t prop;
         QR_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
         b->ncu = prop.multiProcessorCount;
         const int maxsh = int(stage_bytes(kStageMaxGLean, 10, nullptr));
-        const void *kerns[kClasses] = {
-            reinterpret_cast<const void *>(k_batch<true, 1>),
-            reinterpret_cast<const void *>(k_batch<true, 2>),
-            reinterpret_cast<const void *>(k_batch<true, kStageMaxChunks>),
-            reinterpret_cast<const void *>(k_batch<true, kStageLeanChunks>)};
+        const void *kerns[2 * kClasses] = {
+            reinterpret_cast<const void *>(k_batch<true, 1, true>),
+            reinterpret_cast<const void *>(k_batch<true, 2, true>),
+            reinterpret_cast<const void *>(k_batch<true, kStageMaxChunks, true>),
+            reinterpret_cast<const void *>(k_batch<true, kStageLeanChunks, true>),
+            reinterpret_cast<const void *>(k_batch<true, 1, false>),
+            reinterpret_cast<const void *>(k_batch<true, 2, false>),
+            reinterpret_cast<const void *>(k_batch<true, kStageMaxChunks, false>),
+            reinterpret_cast<const void *>(k_batch<true, kStageLeanChunks, false>)};
         for (const void *kf : kerns)
             QR_HIP_CHECK(hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, maxsh));
     }
@@ -236,6 +353,16 @@ int prepare(qr_batch *b, const Plan &plan) {
     }
     if (!plan.desc.empty())
         QR_HIP_CHECK(hipMemcpy(b->d_desc, plan.desc.data(), plan.desc.size() * sizeof(BatchOpDesc),
+                               hipMemcpyHostToDevice));
+    if (b->d_mv_cap < plan.mv.size()) {
+        if (b->d_mv) (void)hipFree(b->d_mv);
+        b->d_mv = nullptr;
+        b->d_mv_cap = 0;
+        QR_HIP_CHECK(hipMalloc(&b->d_mv, plan.mv.size() * sizeof(BatchMvDesc)));
+        b->d_mv_cap = plan.mv.size();
+    }
+    if (!plan.mv.empty())
+        QR_HIP_CHECK(hipMemcpy(b->d_mv, plan.mv.data(), plan.mv.size() * sizeof(BatchMvDesc),
                                hipMemcpyHostToDevice));
     return QR_OK;
 }
@@ -264,6 +391,7 @@ std::vector<uintptr_t> key_of(const qr_batch *b, uint64_t *const *d_indptr,
     k.push_back(tu.tiles_per_item);
     k.push_back(tu.blocks_per_cu);
     k.push_back(tu.order);
+    k.push_back(tu.split);
     return k;
 }
 
@@ -290,6 +418,7 @@ int qr_batch_destroy(qr_batch *b) {
     if (!b) return QR_OK;
     drop_graph(b);
     if (b->d_desc) (void)hipFree(b->d_desc);
+    if (b->d_mv) (void)hipFree(b->d_mv);
     if (b->d_counter) (void)hipFree(b->d_counter);
     if (b->cap) (void)hipStreamDestroy(b->cap);
     delete b;
@@ -302,7 +431,7 @@ int qr_batch_run(qr_batch *b, uint64_t *const *d_indptr, uint64_t *const *d_indi
     if (!b || !d_x || !d_y) return fail(QR_ERR_ARG, "null argument");
     const qr_tuning tu = tuning ? *tuning : (b->have_tuned ? b->tuned : kDefaultTuning);
     if (tu.tiles_per_item == 0 || tu.tiles_per_item > 4096 || tu.blocks_per_cu == 0 ||
-        tu.blocks_per_cu > 8 || tu.order > 1)
+        tu.blocks_per_cu > 8 || tu.order > 1 || tu.split > 1)
         return fail(QR_ERR_ARG, "tuning out of range");
     hipStream_t s = (hipStream_t)stream;
     std::vector<uintptr_t> key = key_of(b, d_indptr, d_indices, d_data, d_x, d_y, tu);
@@ -339,16 +468,21 @@ int qr_batch_tune(qr_batch *b, uint64_t *const *d_indptr, uint64_t *const *d_ind
     const uint32_t tpis[] = {1, 2, 4, 8};
     const uint32_t bpcs[] = {1, 2};
     const uint32_t orders[] = {0, 1};
+    // the split form is a candidate only when every operator keeps its CSR
+    bool can_split = d_indices && d_data;
+    for (size_t k = 0; can_split && k < b->ops.size(); ++k)
+        can_split = d_indices[k] && d_data[k];
     hipEvent_t e0, e1;
     QR_HIP_CHECK(hipEventCreate(&e0));
     QR_HIP_CHECK(hipEventCreate(&e1));
     float best_ms = -1.0f;
     qr_tuning bt = kDefaultTuning;
     int rc = QR_OK;
+    for (uint32_t split = 0; split < (can_split ? 2u : 1u); ++split)
     for (uint32_t order : orders)
         for (uint32_t bpc : bpcs)
             for (uint32_t tpi : tpis) {
-                const qr_tuning tu{tpi, bpc, order, 0};
+                const qr_tuning tu{tpi, bpc, order, split};
                 Plan plan;
                 if ((rc = make_plan(b, d_indptr, d_indices, d_data, d_x, d_y, tu, plan)) ||
                     (rc = prepare(b, plan)))

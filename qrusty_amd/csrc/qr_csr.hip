@@ -222,6 +222,12 @@ int launch_csr_build(const qr_op *op, const DeviceTables *t, uint64_t row_begin,
 // gathers coalesce.  Blocks are handed to the 8 XCDs round-robin; the block
 // index is remapped so that each XCD works on one contiguous range of rows
 // and its L2 sees the neighbouring rows' x lines.
+// CSR stream loads are nontemporal (each entry is read once)
+template <typename T>
+__device__ __forceinline__ T mv_load(const T *p) {
+    return __builtin_nontemporal_load(p);
+}
+
 template <int LPR>
 __global__ __launch_bounds__(256) void k_csr_matvec(uint64_t nrows, uint64_t ncols, uint64_t nnz,
                                                     uint32_t nblocks_xcd,
@@ -239,19 +245,19 @@ __global__ __launch_bounds__(256) void k_csr_matvec(uint64_t nrows, uint64_t nco
     if (row < nrows) {
         const uint64_t k0 = min(indptr[row], nnz), k1 = min(indptr[row + 1], nnz);
         uint64_t k = k0 + lane;
-        // four independent loads in flight per lane
-        for (; k + 3 * LPR < k1; k += 4 * LPR) {
-            uint64_t c[4];
-            double2 a[4];
+        // eight independent entries in flight per lane (measured best of 2, 4, 8)
+        constexpr int U = 8;
+        for (; k + (U - 1) * LPR < k1; k += U * LPR) {
+            uint64_t c[U];
+            double2 a[U];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                c[u] = __builtin_nontemporal_load(indices + k + u * LPR);
-                const dvec2 av = __builtin_nontemporal_load(
-                    reinterpret_cast<const dvec2 *>(data + k + u * LPR));
+            for (int u = 0; u < U; ++u) {
+                c[u] = mv_load(indices + k + u * LPR);
+                const dvec2 av = mv_load(reinterpret_cast<const dvec2 *>(data + k + u * LPR));
                 a[u] = make_double2(av.x, av.y);
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < U; ++u) {
                 double2 bx = make_double2(0.0, 0.0);
                 if (c[u] < ncols) bx = x[c[u]];
                 acc.x = fma(a[u].x, bx.x, acc.x);
@@ -261,8 +267,8 @@ __global__ __launch_bounds__(256) void k_csr_matvec(uint64_t nrows, uint64_t nco
             }
         }
         for (; k < k1; k += LPR) {
-            const uint64_t c = __builtin_nontemporal_load(indices + k);
-            const dvec2 av = __builtin_nontemporal_load(reinterpret_cast<const dvec2 *>(data + k));
+            const uint64_t c = mv_load(indices + k);
+            const dvec2 av = mv_load(reinterpret_cast<const dvec2 *>(data + k));
             double2 bx = make_double2(0.0, 0.0);
             if (c < ncols) bx = x[c];
             acc.x = fma(av.x, bx.x, acc.x);
@@ -283,7 +289,8 @@ int launch_csr_matvec(uint64_t nrows, uint64_t ncols, uint64_t nnz, const uint64
                       const uint64_t *d_indices, const double *d_data, const double *d_x,
                       double *d_y, hipStream_t s) {
     const uint64_t avg = nrows ? nnz / nrows : 0;
-    // ~4+ entries per lane, at most 8 lanes per row (8 rows per wave)
+    // ~4+ entries per lane, at most 8 lanes per row (8 rows per wave; 4 and 16
+    // to 32 lanes per row measured slower at G = 2000)
     int lpr = 1;
     while (lpr < 8 && uint64_t(lpr) * 4 <= avg) lpr *= 2;
     const uint64_t threads = nrows * uint64_t(lpr);
