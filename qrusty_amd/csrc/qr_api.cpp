@@ -125,16 +125,34 @@ void build_patterns(const qr_op *op, int L, int tb, uint32_t pcap, uint32_t ucap
     out = HostPatterns{};
     out.ptptr.push_back(0);
     ApplyChunk cur{};
-    auto close_chunk = [&out, &cur]() {
-        cur.g1 = uint32_t(out.grp.size());
+    const uint64_t far_mask = ~mask;
+    // closes the chunk: its groups are reordered by class (patterns stay where
+    // they are; the table rows follow their groups)
+    auto close_chunk = [&out, &cur, far_mask]() {
+        const uint32_t g0 = cur.cg[0], g1 = uint32_t(out.grp.size());
         cur.p1 = uint32_t(out.pzl.size());
         cur.u1 = uint32_t(out.ug.size());
-        if (cur.g1 > cur.g0) {
+        if (g1 > g0) {
+            auto cls = [far_mask](const ApplyGroup &a) {
+                return ((a.m & far_mask) ? 1u : 0u) | ((a.u >= 0 || a.ps != a.pk) ? 2u : 0u) |
+                       (a.real ? 0u : 4u);
+            };
+            std::stable_sort(out.grp.begin() + g0, out.grp.end(),
+                             [&cls](const ApplyGroup &a, const ApplyGroup &b) { return cls(a) < cls(b); });
+            uint32_t c = 0;
+            for (uint32_t g = g0; g <= g1; ++g) {
+                const uint32_t cg = g < g1 ? cls(out.grp[g]) : uint32_t(kApplyClasses);
+                while (c <= cg && c <= uint32_t(kApplyClasses)) cur.cg[c++] = g;
+                if (g < g1 && out.grp[g].u >= 0) out.ug[cur.u0 + uint32_t(out.grp[g].u)] = g;
+            }
             out.chunk.push_back(cur);
             out.max_p = std::max(out.max_p, cur.p1 - cur.p0);
             out.max_u = std::max(out.max_u, cur.u1 - cur.u0);
         }
-        cur = ApplyChunk{cur.g1, 0, cur.p1, 0, cur.u1, 0, 0, 0};
+        cur = ApplyChunk{};
+        cur.cg[0] = g1;
+        cur.p0 = uint32_t(out.pzl.size());
+        cur.u0 = uint32_t(out.ug.size());
     };
     auto cls = [tmask](uint64_t zl) {
         if ((zl & ~tmask) == 0) return 0;  // thread bits only (or zl == 0)

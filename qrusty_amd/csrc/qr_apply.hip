@@ -90,69 +90,108 @@ __device__ __forceinline__ double flip_sign(double a, uint32_t flip) {
     return __hiloint2double(__double2hiint(a) ^ int(flip), __double2loint(a));
 }
 
+// Word 3 of a raw (untyped, stride 0) buffer resource on gfx9: 32-bit data
+// format; offsets are bounds-checked against num_records bytes.
+constexpr int kBufferWord3 = 0x00020000;
+
 // Gathers of x[r ^ m] for the R rows of a thread: from the LDS x tile, or --
 // for masks above the tile -- from the partner tile in global memory, with
 // one uniform base pointer per row block and the thread's offset shared.
-template <bool kFull, int R, typename cT>
+template <bool kFar, bool kFull, int R, typename cT>
 __device__ __forceinline__ void gather(uint64_t m, uint32_t tid, uint64_t tile_rows, uint64_t base,
                                        const cT *xt, const cT *__restrict__ xv, cT (&xg)[R]) {
     const uint64_t mlo = m & (tile_rows - 1), mhi = m & ~(tile_rows - 1);
     const uint32_t tx = tid ^ uint32_t(mlo & (kApplyThreads - 1));
     const uint32_t mk = uint32_t(mlo >> kApplyThreadBits);
-    if (mhi) {
-        const cT *__restrict__ far = xv + (base ^ mhi);
+    if (kFar) {
+        // the partner tile through a buffer resource: the row block's offset
+        // is a scalar, the thread's offset one VGPR shared by the R loads
+        const cT *far = xv + (base ^ mhi);
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<cT *>(far), 0, int(tile_rows * sizeof(cT)), kBufferWord3);
+        const uint32_t voff = tx * uint32_t(sizeof(cT));
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             if (!kFull && tid + uint64_t(k) * kApplyThreads >= tile_rows) break;
-            const cT *__restrict__ fk = far + (size_t(uint32_t(k) ^ mk) << kApplyThreadBits);
-            xg[k] = fk[tx];
+            const uint32_t soff = ((uint32_t(k) ^ mk) << kApplyThreadBits) * uint32_t(sizeof(cT));
+            if constexpr (sizeof(cT) == 8)
+                xg[k] = __builtin_bit_cast(cT, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 0));
+            else
+                xg[k] = __builtin_bit_cast(cT, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, 0));
         }
     } else {
+        // the thread's column in the LDS tile once, then a uniform row offset
+        const cT *xrow = xt + tx;
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             if (!kFull && tid + uint64_t(k) * kApplyThreads >= tile_rows) break;
-            xg[k] = xt[tx + ((uint32_t(k) ^ mk) << kApplyThreadBits)];
+            xg[k] = xrow[(uint32_t(k) ^ mk) << kApplyThreadBits];
         }
     }
 }
 
 // y += D_g(r) x[r ^ m] for the R rows of a thread, x already gathered.  D_g is
 // formed from the chunk's pattern coefficients: thread-bit patterns once per
-// thread, then -- only when the group has them -- per row the row-bit table
-// entry sU[urow*R + k] and the straddling patterns.  Products go to the FP64
-// accumulators ar/ai, or (mixed) to the FP32 block partials br/bi.
-template <bool kMixed, bool kReal, bool kFull, int R, typename cT>
+// thread, then -- for row-varying groups (kRowvar) -- per row the row-bit
+// table entry sU[urow*R + k] and the straddling patterns.  Products go to the
+// FP64 accumulators ar/ai, or (mixed) to the FP32 block partials br/bi; those
+// with Im D only for complex groups (kCplx).
+template <bool kMixed, bool kFull, bool kRowvar, bool kCplx, int R, typename cT>
 __device__ __forceinline__ void accumulate(const ApplyGroup &gd, uint32_t tid, uint64_t tile_rows,
                                            uint32_t zero_row, const double2 *sA, const uint32_t *sZ,
                                            const double2 *sU, const cT (&xg)[R], double (&ar)[R],
                                            double (&ai)[R], float (&br)[R], float (&bi)[R]) {
+    constexpr bool cplx = kCplx;
     double cr = 0.0, ci = 0.0;
-    for (uint32_t p = gd.p0; p < gd.ps; ++p) {
+    uint32_t p = gd.p0;
+    for (; p + 1 < gd.ps; p += 2) {  // two patterns per step: loads in flight together
+        const double2 a0 = sA[p], a1 = sA[p + 1];
+        const uint32_t f0 = uint32_t(__popc(tid & sZ[p])) << 31;
+        const uint32_t f1 = uint32_t(__popc(tid & sZ[p + 1])) << 31;
+        cr += flip_sign(a0.x, f0);
+        cr += flip_sign(a1.x, f1);
+        if (cplx) {
+            ci += flip_sign(a0.y, f0);
+            ci += flip_sign(a1.y, f1);
+        }
+    }
+    if (p < gd.ps) {
         const double2 a = sA[p];
         const uint32_t flip = uint32_t(__popc(tid & sZ[p])) << 31;
         cr += flip_sign(a.x, flip);
-        if (!kReal) ci += flip_sign(a.y, flip);
+        if (cplx) ci += flip_sign(a.y, flip);
     }
-    if (gd.u < 0 && gd.ps == gd.pk) {
+    if (!kRowvar) {
         // D is the same for the R rows of this thread
-        const float fr = float(cr), fi = float(ci);
+        if (kMixed) {
+            const float fr = float(cr), fi = float(ci);
 #pragma unroll
-        for (int k = 0; k < R; ++k) {
-            if (!kFull && tid + uint64_t(k) * kApplyThreads >= tile_rows) break;
-            const cT xb = xg[k];
-            if (!kMixed) {
-                ar[k] = fma(cr, xb.x, ar[k]);
-                ai[k] = fma(cr, xb.y, ai[k]);
-                if (!kReal) {
-                    ar[k] = fma(-ci, xb.y, ar[k]);
-                    ai[k] = fma(ci, xb.x, ai[k]);
+            for (int k = 0; k < R; ++k) {
+                if (!kFull && tid + uint64_t(k) * kApplyThreads >= tile_rows) break;
+                br[k] = fmaf(fr, xg[k].x, br[k]);
+                bi[k] = fmaf(fr, xg[k].y, bi[k]);
+            }
+            if (cplx) {
+#pragma unroll
+                for (int k = 0; k < R; ++k) {
+                    if (!kFull && tid + uint64_t(k) * kApplyThreads >= tile_rows) break;
+                    br[k] = fmaf(-fi, xg[k].y, br[k]);
+                    bi[k] = fmaf(fi, xg[k].x, bi[k]);
                 }
-            } else {
-                br[k] = fmaf(fr, xb.x, br[k]);
-                bi[k] = fmaf(fr, xb.y, bi[k]);
-                if (!kReal) {
-                    br[k] = fmaf(-fi, xb.y, br[k]);
-                    bi[k] = fmaf(fi, xb.x, bi[k]);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                if (!kFull && tid + uint64_t(k) * kApplyThreads >= tile_rows) break;
+                ar[k] = fma(cr, xg[k].x, ar[k]);
+                ai[k] = fma(cr, xg[k].y, ai[k]);
+            }
+            if (cplx) {
+#pragma unroll
+                for (int k = 0; k < R; ++k) {
+                    if (!kFull && tid + uint64_t(k) * kApplyThreads >= tile_rows) break;
+                    ar[k] = fma(-ci, xg[k].y, ar[k]);
+                    ai[k] = fma(ci, xg[k].x, ai[k]);
                 }
             }
         }
@@ -163,20 +202,20 @@ __device__ __forceinline__ void accumulate(const ApplyGroup &gd, uint32_t tid, u
     for (int k = 0; k < R; ++k) {
         if (!kFull && tid + uint64_t(k) * kApplyThreads >= tile_rows) break;
         const double2 uk = su[k];
-        double dr = cr + uk.x, di = kReal ? 0.0 : ci + uk.y;
-        for (uint32_t p = gd.ps; p < gd.pk; ++p) {
-            const double2 a = sA[p];
-            const uint32_t zl = uint32_t(__builtin_amdgcn_readfirstlane(int(sZ[p])));
+        double dr = cr + uk.x, di = cplx ? ci + uk.y : 0.0;
+        for (uint32_t q = gd.ps; q < gd.pk; ++q) {
+            const double2 a = sA[q];
+            const uint32_t zl = uint32_t(__builtin_amdgcn_readfirstlane(int(sZ[q])));
             const uint32_t flip =
                 uint32_t(__popc(tid & zl) + __popc(uint32_t(k) & (zl >> kApplyThreadBits))) << 31;
             dr += flip_sign(a.x, flip);
-            if (!kReal) di += flip_sign(a.y, flip);
+            if (cplx) di += flip_sign(a.y, flip);
         }
         const cT xb = xg[k];
         if (!kMixed) {
             ar[k] = fma(dr, xb.x, ar[k]);
             ai[k] = fma(dr, xb.y, ai[k]);
-            if (!kReal) {
+            if (cplx) {
                 ar[k] = fma(-di, xb.y, ar[k]);
                 ai[k] = fma(di, xb.x, ai[k]);
             }
@@ -184,7 +223,7 @@ __device__ __forceinline__ void accumulate(const ApplyGroup &gd, uint32_t tid, u
             const float fr = float(dr);
             br[k] = fmaf(fr, xb.x, br[k]);
             bi[k] = fmaf(fr, xb.y, bi[k]);
-            if (!kReal) {
+            if (cplx) {
                 const float fi = float(di);
                 br[k] = fmaf(-fi, xb.y, br[k]);
                 bi[k] = fmaf(fi, xb.x, bi[k]);
@@ -285,27 +324,17 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_tiled(
         sr[k] = si[k] = cr[k] = ci[k] = br[k] = bi[k] = 0.0f;
     }
 
-    int since_flush = 0;
-    // one group: accumulate, and every kCompBlock groups add the FP32 block
-    // partials into the running sums with TwoSum
-    auto step = [&](const ApplyGroup &gd, const cT (&xg)[R]) {
-        if (gd.real)
-            accumulate<kMixed, true, kFull, R>(gd, tid, tile_rows, zero_row, sA, sZ, sU, xg, ar, ai,
-                                               br, bi);
-        else
-            accumulate<kMixed, false, kFull, R>(gd, tid, tile_rows, zero_row, sA, sZ, sU, xg, ar,
-                                                ai, br, bi);
-        if (kMixed && ++since_flush == kCompBlock) {
-            since_flush = 0;
+    uint32_t since_flush = 0;  // groups in the FP32 block partials
+    // add the FP32 block partials into the running sums with TwoSum
+    auto flush = [&]() {
 #pragma unroll
-            for (int k = 0; k < R; ++k) {
-                float e;
-                two_sum(sr[k], br[k], sr[k], e);
-                cr[k] += e;
-                two_sum(si[k], bi[k], si[k], e);
-                ci[k] += e;
-                br[k] = bi[k] = 0.0f;
-            }
+        for (int k = 0; k < R; ++k) {
+            float e;
+            two_sum(sr[k], br[k], sr[k], e);
+            cr[k] += e;
+            two_sum(si[k], bi[k], si[k], e);
+            ci[k] += e;
+            br[k] = bi[k] = 0.0f;
         }
     };
     cT xg[R];
@@ -336,22 +365,46 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_tiled(
             sU[(u < ch.u1 - ch.u0 ? u : zero_row) * R + k] = make_double2(s_r, s_i);
         }
         __syncthreads();
-        for (uint32_t g = ch.g0; g < ch.g1; ++g) {
-            const ApplyGroup gd = grp[g];
-            gather<kFull, R>(gd.m, tid, tile_rows, base, xt, xv, xg);
-            step(gd, xg);
-        }
+        // one loop per group class, each with a single straight path, in
+        // blocks of at most kCompBlock groups flushed after each block
+        auto run_class = [&](auto far_t, auto rowvar_t, auto cplx_t, uint32_t g0, uint32_t g1) {
+            constexpr bool kFar = decltype(far_t)::value, kRowvar = decltype(rowvar_t)::value;
+            constexpr bool kCplx = decltype(cplx_t)::value;
+            for (uint32_t gb = g0; gb < g1;) {
+                // up to the next flush point, counted over all classes
+                const uint32_t room = uint32_t(kCompBlock) - since_flush;
+                const uint32_t ge = g1 - gb > room ? gb + room : g1;
+                for (uint32_t g = gb; g < ge; ++g) {
+                    const ApplyGroup gd = grp[g];
+                    // all descriptor fields in SGPRs before the gathers are issued,
+                    // so that no later scalar-load wait also waits for them
+                    asm volatile("" ::"s"(gd.p0), "s"(gd.ps), "s"(gd.pk), "s"(gd.p1), "s"(gd.u));
+                    gather<kFar, kFull, R>(gd.m, tid, tile_rows, base, xt, xv, xg);
+                    accumulate<kMixed, kFull, kRowvar, kCplx, R>(gd, tid, tile_rows, zero_row, sA, sZ,
+                                                                 sU, xg, ar, ai, br, bi);
+                }
+                since_flush += ge - gb;
+                gb = ge;
+                // reset in both precisions: room must stay >= 1 for the loop to advance
+                if (since_flush == uint32_t(kCompBlock)) {
+                    if (kMixed) flush();
+                    since_flush = 0;
+                }
+            }
+        };
+        using F = std::false_type;
+        using T = std::true_type;
+        run_class(F{}, F{}, F{}, ch.cg[0], ch.cg[1]);
+        run_class(T{}, F{}, F{}, ch.cg[1], ch.cg[2]);
+        run_class(F{}, T{}, F{}, ch.cg[2], ch.cg[3]);
+        run_class(T{}, T{}, F{}, ch.cg[3], ch.cg[4]);
+        run_class(F{}, F{}, T{}, ch.cg[4], ch.cg[5]);
+        run_class(T{}, F{}, T{}, ch.cg[5], ch.cg[6]);
+        run_class(F{}, T{}, T{}, ch.cg[6], ch.cg[7]);
+        run_class(T{}, T{}, T{}, ch.cg[7], ch.cg[8]);
     }
-    if (kMixed && since_flush) {
-#pragma unroll
-        for (int k = 0; k < R; ++k) {
-            float e;
-            two_sum(sr[k], br[k], sr[k], e);
-            cr[k] += e;
-            two_sum(si[k], bi[k], si[k], e);
-            ci[k] += e;
-        }
-    }
+
+    if (kMixed && since_flush) flush();
 
     double er = 0.0, ei = 0.0;
     cT *__restrict__ yv = yout ? reinterpret_cast<cT *>(yout) + uint64_t(v) * N : nullptr;

@@ -45,10 +45,16 @@ struct ApplyGroup {
     uint32_t real;  // every coefficient of the group is real
 };
 
-// Groups [g0, g1) whose patterns [p0, p1) and table rows [u0, u1) are staged
-// in LDS together.
+// Groups whose patterns [p0, p1) and table rows [u0, u1) are staged in LDS
+// together.  Inside a chunk the groups are ordered by class -- bit 0: the mask
+// reaches outside the tile ("far"), bit 1: D varies by row (row-bit or
+// straddling patterns), bit 2: complex coefficients -- and class c holds the
+// groups [cg[c], cg[c+1]).
+constexpr int kApplyClasses = 8;
 struct ApplyChunk {
-    uint32_t g0, g1, p0, p1, u0, u1, pad0, pad1;
+    uint32_t cg[kApplyClasses + 1];
+    uint32_t p0, p1, u0, u1;
+    uint32_t pad[3];
 };
 
 // Device copies of an operator's tables (one set per device that used it).
