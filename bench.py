@@ -374,6 +374,59 @@ def bench_large_apply(args, torch, qa):
     return res
 
 
+def bench_vec(args, torch, qa):
+    """Eigensolver-step operations on the device: axpby and precond2 on
+    2^27-element complex128 vectors (48 B per element: two reads, one write),
+    and SpMat.diagonal of the 24-qubit Heisenberg CSR (2^24 rows, G = 25)."""
+    from qrusty_amd import _lib as L
+    from qrusty_amd import workloads
+
+    n = 1 << 27
+    g = torch.Generator(device="cuda")
+    g.manual_seed(7)
+    x = torch.randn(n, dtype=torch.complex128, device="cuda", generator=g)
+    y = torch.randn(n, dtype=torch.complex128, device="cuda", generator=g)
+    z = torch.empty_like(x)
+    stream = torch.cuda.current_stream()
+    sh = ctypes.c_void_p(stream.cuda_stream)
+    P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    a = (ctypes.c_double * 2)(0.3, -1.7)
+    b = (ctypes.c_double * 2)(-2.5, 0.25)
+    e = (ctypes.c_double * 2)(0.25, -0.5)
+
+    def timed(fn, reps):
+        fn()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        torch.cuda.synchronize()
+        ev[0].record(stream)
+        for _ in range(reps):
+            fn()
+        ev[1].record(stream)
+        torch.cuda.synchronize()
+        return ev[0].elapsed_time(ev[1]) / reps
+
+    reps = max(3, args.steps)
+    t_axpby = timed(lambda: L.check(L.lib.qr_axpby(n, a, P(x), b, P(y), P(z), sh)), reps)
+    t_pre = timed(lambda: L.check(L.lib.qr_precond2(n, P(x), P(y), e, 1e-14, P(z), sh)), reps)
+    del x, y, z
+    labels, coeffs = workloads.heisenberg_ring(24, seed=workloads.SEED_CFG3)
+    m = qa.SparsePauliOp([qa.Pauli(l) for l in labels], coeffs).to_matrix_mode("Hip")
+    ip, ix, dv = m.device_arrays()
+    N = 1 << 24
+    d = torch.empty(N, dtype=torch.complex128, device="cuda")
+    t_diag = timed(lambda: L.check(L.lib.qr_csr_diagonal(N, N, dv.numel(), P(ip), P(ix), P(dv),
+                                                         P(d), sh)), reps)
+    res = {"workload": "axpby / precond2 on 2^27 complex128; diagonal of the 24-qubit Heisenberg CSR",
+           "axpby_ms": t_axpby, "axpby_GBps": 48 * n / t_axpby / 1e6,
+           "precond2_ms": t_pre, "precond2_GBps": 48 * n / t_pre / 1e6,
+           "diagonal_ms": t_diag, "diagonal_rows_per_s": N / (t_diag / 1e3)}
+    log(f"vec: axpby {t_axpby:.3f} ms ({res['axpby_GBps']:.0f} GB/s), precond2 {t_pre:.3f} ms "
+        f"({res['precond2_GBps']:.0f} GB/s), diagonal {t_diag:.3f} ms ({res['diagonal_rows_per_s']:.3e} rows/s)")
+    del m, ip, ix, dv, d
+    torch.cuda.empty_cache()
+    return res
+
+
 def bench_copy(args, torch):
     """Plain device-to-device copy of 2 GiB (torch copy_): the calibration
     kernel whose known bytes scale the HBM counters FETCH_SIZE / WRITE_SIZE."""
@@ -404,7 +457,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the Heisenberg apply workload")
     ap.add_argument("--no-check", action="store_true", help="skip the sampled parity check")
-    ap.add_argument("--only", choices=["csr", "apply", "batch", "large", "copy"], default=None)
+    ap.add_argument("--only", choices=["csr", "apply", "batch", "large", "vec", "copy"], default=None)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
@@ -447,6 +500,8 @@ def main():
         extra["batch"] = bench_batch(args, torch, qa)
     if args.only in (None, "large") and not args.no_extra:
         extra["large_apply"] = bench_large_apply(args, torch, qa)
+    if args.only in (None, "vec") and not args.no_extra:
+        extra["vec"] = bench_vec(args, torch, qa)
     out["extra"] = extra
     print(json.dumps(out), flush=True)
 

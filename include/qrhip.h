@@ -146,6 +146,34 @@ int qr_csr_matvec(uint64_t nrows, uint64_t ncols, uint64_t nnz, const uint64_t *
                   double *d_y, void *stream);
 
 /* ------------------------------------------------------------------------
+ * Eigensolver-step operations on device vectors (complex128, interleaved).
+ * Arithmetic is num-complex's: no fused multiply-add; division
+ * ((a c + b d) / |q|^2, (b c - a d) / |q|^2); norm = hypot.
+ *  qr_csr_diagonal : out[r] = the entry (r, r) of a CSR with ascending columns
+ *                    per row, 0 where absent; r < min(nrows, ncols).  Replaces
+ *                    SpMat.diagonal (pyqrusty/src/lib.rs:118-125).
+ *  qr_axpby / qr_axpy / qr_ax : z = a x + b y, a x + y, a x; a and b are host
+ *                    complex128 scalars (2 doubles).  Replace
+ *                    qrusty::accel::axpby / axpy / ax (accel.rs:374-393).
+ *  qr_precond2     : out = dx / reg(diag - e, tol), reg(v) = tol + 0i when
+ *                    |v| < tol, else v (pyqrusty/src/lib.rs:436-468).
+ *  qr_precond      : the same with diag read from a CSR (length min(nrows,
+ *                    ncols)).
+ * ---------------------------------------------------------------------- */
+int qr_csr_diagonal(uint64_t nrows, uint64_t ncols, uint64_t nnz, const uint64_t *d_indptr,
+                    const uint64_t *d_indices, const double *d_data, double *d_out, void *stream);
+int qr_axpby(uint64_t n, const double *a, const double *d_x, const double *b, const double *d_y,
+             double *d_z, void *stream);
+int qr_axpy(uint64_t n, const double *a, const double *d_x, const double *d_y, double *d_z,
+            void *stream);
+int qr_ax(uint64_t n, const double *a, const double *d_x, double *d_z, void *stream);
+int qr_precond(uint64_t nrows, uint64_t ncols, uint64_t nnz, const uint64_t *d_indptr,
+               const uint64_t *d_indices, const double *d_data, const double *d_dx,
+               const double *e, double tol, double *d_out, void *stream);
+int qr_precond2(uint64_t n, const double *d_diag, const double *d_dx, const double *e, double tol,
+                double *d_out, void *stream);
+
+/* ------------------------------------------------------------------------
  * Matrix-free Pauli-sum apply (no counterpart in the reference; equal to the
  * CSR build followed by the CSR matvec):
  *   y_v[r] = sum_g D_g(r) * x_v[r ^ x_g],  D_g(r) = sum_{t in g} (-1)^popcount(r&z_t) c'_t

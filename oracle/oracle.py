@@ -72,6 +72,12 @@ def lib():
         L.orc_apply_grouped.restype = None
         L.orc_vdot.argtypes = [ctypes.c_uint64, ctypes.c_uint32, f64p, f64p, f64p]
         L.orc_vdot.restype = None
+        L.orc_csr_diagonal.argtypes = [ctypes.c_uint64, ctypes.c_uint64, u64p, u64p, f64p, f64p]
+        L.orc_csr_diagonal.restype = None
+        L.orc_axpby.argtypes = [ctypes.c_uint64, ctypes.c_int, f64p, f64p, f64p, f64p, f64p]
+        L.orc_axpby.restype = None
+        L.orc_precond2.argtypes = [ctypes.c_uint64, f64p, f64p, f64p, ctypes.c_double, f64p]
+        L.orc_precond2.restype = None
         _lib = L
     return _lib
 
@@ -298,3 +304,45 @@ def dense_sum(labels, coeffs, convention: str = "rowwise"):
         # each row of a Pauli string has exactly one nonzero: no duplicate (r, c)
         dense[P.row, P.col] = dense[P.row, P.col] + coeffs[t] * vals
     return dense
+
+
+# ---------------------------------------------------------------------------
+# eigensolver-step operations
+# ---------------------------------------------------------------------------
+def _c128(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.complex128))
+
+
+def csr_diagonal(shape, indptr, indices, data):
+    """sprs CsMat::diag().to_dense() (pyqrusty/src/lib.rs:118-125)."""
+    n = min(shape)
+    out = np.zeros(n, dtype=np.complex128)
+    ip = np.ascontiguousarray(indptr, dtype=np.uint64)
+    ix = np.ascontiguousarray(indices, dtype=np.uint64)
+    d = _c128(data)
+    lib().orc_csr_diagonal(shape[0], shape[1], _p(ip, u64p), _p(ix, u64p),
+                           _p(d.view(np.float64), f64p), _p(out.view(np.float64), f64p))
+    return out
+
+
+def axpby(a, x, b, y, mode: int = 0):
+    """mode 0: a x + b y; 1: a x + y; 2: a x (qrusty/src/accel.rs:374-393)."""
+    x = _c128(x)
+    y = _c128(y) if y is not None else x
+    z = np.empty_like(x)
+    ab = np.array([complex(a).real, complex(a).imag, complex(b).real, complex(b).imag])
+    lib().orc_axpby(len(x), mode, _p(ab[:2], f64p), _p(x.view(np.float64), f64p),
+                    _p(np.ascontiguousarray(ab[2:]), f64p), _p(y.view(np.float64), f64p),
+                    _p(z.view(np.float64), f64p))
+    return z
+
+
+def precond2(diag, dx, e, tol):
+    """dx / reg(diag - e, tol) (pyqrusty/src/lib.rs:436-468)."""
+    d = _c128(diag)
+    dxa = _c128(dx)
+    out = np.empty_like(dxa)
+    ev = np.array([complex(e).real, complex(e).imag])
+    lib().orc_precond2(len(d), _p(d.view(np.float64), f64p), _p(dxa.view(np.float64), f64p),
+                       _p(ev, f64p), float(tol), _p(out.view(np.float64), f64p))
+    return out

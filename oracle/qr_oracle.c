@@ -21,7 +21,16 @@
  *                          bit order inside sprs dot_dense is third-party code
  *                          that is absent here: "parity unpinned" at bit level
  *   orc_apply_rows         equals CSR build + matvec (checked on small cases)
+ *   orc_csr_diagonal       numpy.diagonal of the dense matrix; test_it.py:46-51
+ *   orc_axpby              bit-exact against CPython complex * and + (IEEE
+ *                          double, no contraction), allclose as test_it.py:232-263
+ *   orc_precond2           CPython complex division (Smith's method) within a
+ *                          few ulp; reg threshold cases; the H2 vector printed
+ *                          in test_it.py:285-300 (to its 9 digits).  The bit
+ *                          order of num-complex's division is not available
+ *                          here: "parity unpinned" at the last bit
  */
+#include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -493,5 +502,75 @@ void orc_vdot(uint64_t N, uint32_t nvec, const double *psi, const double *y, dou
         }
         out[2 * v] = re;
         out[2 * v + 1] = im;
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * Eigensolver-step operations (pyqrusty/src/lib.rs:118-125, 436-468;
+ * qrusty/src/accel.rs:374-393).  num-complex 0.4 arithmetic: Complex * is
+ * (a.re x.re - a.im x.im, a.re x.im + a.im x.re); Complex / is
+ * ((a c + b d) / (c c + d d), (b c - a d) / (c c + d d)); norm() is hypot.
+ * ------------------------------------------------------------------------- */
+
+/* sprs CsMat::diag().to_dense(): entry (r, r) of each row, 0 where absent;
+ * rows hold ascending columns (a valid CsMat), searched in order here. */
+void orc_csr_diagonal(uint64_t nrows, uint64_t ncols, const uint64_t *indptr,
+                      const uint64_t *indices, const double *data, double *out)
+{
+    uint64_t n = nrows < ncols ? nrows : ncols;
+    for (uint64_t r = 0; r < n; ++r) {
+        out[2 * r] = 0.0;
+        out[2 * r + 1] = 0.0;
+        for (uint64_t k = indptr[r]; k < indptr[r + 1]; ++k) {
+            if (indices[k] == r) {
+                out[2 * r] = data[2 * k];
+                out[2 * r + 1] = data[2 * k + 1];
+                break;
+            }
+        }
+    }
+}
+
+/* accel.rs:374-393: mode 0 z = a x + b y, mode 1 z = a x + y, mode 2 z = a x */
+void orc_axpby(uint64_t n, int mode, const double *a, const double *x, const double *b,
+               const double *y, double *z)
+{
+    for (uint64_t i = 0; i < n; ++i) {
+        double axr = a[0] * x[2 * i] - a[1] * x[2 * i + 1];
+        double axi = a[0] * x[2 * i + 1] + a[1] * x[2 * i];
+        if (mode == 2) {
+            z[2 * i] = axr;
+            z[2 * i + 1] = axi;
+            continue;
+        }
+        double byr, byi;
+        if (mode == 0) {
+            byr = b[0] * y[2 * i] - b[1] * y[2 * i + 1];
+            byi = b[0] * y[2 * i + 1] + b[1] * y[2 * i];
+        } else {
+            byr = y[2 * i];
+            byi = y[2 * i + 1];
+        }
+        z[2 * i] = axr + byr;
+        z[2 * i + 1] = axi + byi;
+    }
+}
+
+/* pyqrusty/src/lib.rs:436-468: out = dx / reg(diag - e, tol), reg(v) = tol + 0i
+ * when |v| < tol, else v */
+void orc_precond2(uint64_t n, const double *diag, const double *dx, const double *e, double tol,
+                  double *out)
+{
+    for (uint64_t i = 0; i < n; ++i) {
+        double vr = diag[2 * i] - e[0], vi = diag[2 * i + 1] - e[1];
+        if (hypot(vr, vi) < tol) {
+            vr = tol;
+            vi = 0.0;
+        }
+        double nn = vr * vr + vi * vi;
+        double re = dx[2 * i] * vr + dx[2 * i + 1] * vi;
+        double im = dx[2 * i + 1] * vr - dx[2 * i] * vi;
+        out[2 * i] = re / nn;
+        out[2 * i + 1] = im / nn;
     }
 }

@@ -31,7 +31,8 @@ import numpy as np
 from . import _lib as L
 from ._lib import QrError
 
-__all__ = ["Pauli", "SparsePauliOp", "SpMat", "spmat_dot_densevec", "csr_matrix", "Batch", "QrError"]
+__all__ = ["Pauli", "SparsePauliOp", "SpMat", "spmat_dot_densevec", "csr_matrix", "axpby", "axpy",
+           "ax", "precond", "precond2", "Batch", "QrError"]
 __version__ = "0.1.0"
 
 _PRECISIONS = {"fp64": L.QR_PREC_FP64, "mixed": L.QR_PREC_MIXED}
@@ -350,18 +351,17 @@ class SpMat:
         self._live("cannot get NNZ of an exported sparse matrix")
         return int(self._data.numel())
 
-    def diagonal(self):
-        """Dense main diagonal (pyqrusty/src/lib.rs:118-125), computed from a host copy."""
+    def diagonal(self, device: bool = False):
+        """Dense main diagonal (pyqrusty/src/lib.rs:118-125): entry (r, r) of each
+        row, 0 where absent, computed by qr_csr_diagonal on the device.  Returns
+        numpy complex128 like the reference, or the device tensor with device=True."""
         self._live("SpMat.diagonal(): already-exported sparse matrix")
-        indptr = self._indptr.cpu().numpy().view(np.uint64)
-        indices = self._indices.cpu().numpy().view(np.uint64)
-        data = self._data.cpu().numpy()
-        n = min(self._shape)
-        out = np.zeros(n, dtype=np.complex128)
-        rows = np.repeat(np.arange(self._shape[0], dtype=np.uint64), np.diff(indptr).astype(np.int64))
-        hit = (rows == indices) & (rows < n)
-        out[rows[hit].astype(np.int64)] = data[hit]
-        return out
+        torch = _require_gpu()
+        out = torch.empty(min(self._shape), dtype=torch.complex128, device=self._data.device)
+        L.check(L.lib.qr_csr_diagonal(self._shape[0], self._shape[1], self._data.numel(),
+                                      _ptr(self._indptr), _ptr(self._indices), _ptr(self._data),
+                                      _ptr(out), _stream_handle(torch)))
+        return out if device else out.cpu().numpy()
 
     def export(self):
         """((rows, cols), data complex128, indices uint64, indptr uint64); the
@@ -423,6 +423,95 @@ def spmat_dot_densevec(spmat: SpMat, x):
                                 _ptr(spmat._indices), _ptr(spmat._data), _ptr(xt), _ptr(y),
                                 _stream_handle(torch)))
     return y.cpu().numpy() if is_np else y
+
+
+# ---------------------------------------------------------------------------
+# eigensolver-step operations (pyqrusty/src/lib.rs:436-468, 519-585;
+# qrusty/src/accel.rs:374-393)
+# ---------------------------------------------------------------------------
+def _vec_in(torch, x, name):
+    """A 1-D complex128 device tensor for x, and whether x was numpy."""
+    if isinstance(x, torch.Tensor):
+        if x.dtype != torch.complex128 or x.dim() != 1 or not x.is_cuda:
+            raise QrError(f"{name}: expected a 1-D complex128 device tensor")
+        return x.contiguous(), False
+    xa = np.ascontiguousarray(np.asarray(x, dtype=np.complex128))
+    if xa.ndim != 1:
+        raise QrError(f"{name}: expected a one-dimensional array")
+    return torch.from_numpy(xa).to(device="cuda"), True
+
+
+def _scalar(c):
+    c = complex(c)
+    return (ctypes.c_double * 2)(c.real, c.imag)
+
+
+def _vec_out(torch, z, as_np):
+    return z.cpu().numpy() if as_np else z
+
+
+def axpby(a, x, b, y):
+    """z = a x + b y (qrusty/src/accel.rs:374-379)."""
+    torch = _require_gpu()
+    xt, np_in = _vec_in(torch, x, "axpby")
+    yt, _ = _vec_in(torch, y, "axpby")
+    if xt.numel() != yt.numel():
+        raise QrError(f"axpby: length mismatch ({xt.numel()} != {yt.numel()})")
+    z = torch.empty_like(xt)
+    L.check(L.lib.qr_axpby(xt.numel(), _scalar(a), _ptr(xt), _scalar(b), _ptr(yt), _ptr(z),
+                           _stream_handle(torch)))
+    return _vec_out(torch, z, np_in)
+
+
+def axpy(a, x, y):
+    """z = a x + y (qrusty/src/accel.rs:381-386)."""
+    torch = _require_gpu()
+    xt, np_in = _vec_in(torch, x, "axpy")
+    yt, _ = _vec_in(torch, y, "axpy")
+    if xt.numel() != yt.numel():
+        raise QrError(f"axpy: length mismatch ({xt.numel()} != {yt.numel()})")
+    z = torch.empty_like(xt)
+    L.check(L.lib.qr_axpy(xt.numel(), _scalar(a), _ptr(xt), _ptr(yt), _ptr(z),
+                          _stream_handle(torch)))
+    return _vec_out(torch, z, np_in)
+
+
+def ax(a, x):
+    """z = a x (qrusty/src/accel.rs:388-393)."""
+    torch = _require_gpu()
+    xt, np_in = _vec_in(torch, x, "ax")
+    z = torch.empty_like(xt)
+    L.check(L.lib.qr_ax(xt.numel(), _scalar(a), _ptr(xt), _ptr(z), _stream_handle(torch)))
+    return _vec_out(torch, z, np_in)
+
+
+def precond(m: SpMat, dx, e, tol: float):
+    """dx / reg(diag(m) - e, tol) (pyqrusty/src/lib.rs:441-454, 569-579)."""
+    if m._data is None:
+        raise QrError("cannot call precond with an exported sparse matrix")
+    torch = _require_gpu()
+    dxt, np_in = _vec_in(torch, dx, "precond")
+    n = min(m._shape)
+    if dxt.numel() != n:
+        raise QrError(f"precond: dx has length {dxt.numel()}, the diagonal {n}")
+    z = torch.empty_like(dxt)
+    L.check(L.lib.qr_precond(m._shape[0], m._shape[1], m._data.numel(), _ptr(m._indptr),
+                             _ptr(m._indices), _ptr(m._data), _ptr(dxt), _scalar(e), float(tol),
+                             _ptr(z), _stream_handle(torch)))
+    return _vec_out(torch, z, np_in)
+
+
+def precond2(diag, dx, e, tol: float):
+    """dx / reg(diag - e, tol) (pyqrusty/src/lib.rs:456-468, 581-591)."""
+    torch = _require_gpu()
+    dt, np_in = _vec_in(torch, diag, "precond2")
+    dxt, _ = _vec_in(torch, dx, "precond2")
+    if dt.numel() != dxt.numel():
+        raise QrError(f"precond2: length mismatch ({dt.numel()} != {dxt.numel()})")
+    z = torch.empty_like(dxt)
+    L.check(L.lib.qr_precond2(dt.numel(), _ptr(dt), _ptr(dxt), _scalar(e), float(tol), _ptr(z),
+                              _stream_handle(torch)))
+    return _vec_out(torch, z, np_in)
 
 
 def csr_matrix(m: SpMat):

@@ -42,6 +42,12 @@ SIGNATURES = {
     "qr_csr_build_rows": (ctypes.c_int, [vp, u64, u64, vp, vp, vp]),
     "qr_csr_matvec": (ctypes.c_int, [u64, u64, u64, vp, vp, vp, vp, vp, vp]),
     "qr_pauli_apply": (ctypes.c_int, [vp, i32, u32, vp, vp, vp, vp]),
+    "qr_csr_diagonal": (ctypes.c_int, [u64, u64, u64, vp, vp, vp, vp, vp]),
+    "qr_axpby": (ctypes.c_int, [u64, f64p, vp, f64p, vp, vp, vp]),
+    "qr_axpy": (ctypes.c_int, [u64, f64p, vp, vp, vp, vp]),
+    "qr_ax": (ctypes.c_int, [u64, f64p, vp, vp, vp]),
+    "qr_precond": (ctypes.c_int, [u64, u64, u64, vp, vp, vp, vp, f64p, ctypes.c_double, vp, vp]),
+    "qr_precond2": (ctypes.c_int, [u64, vp, vp, f64p, ctypes.c_double, vp, vp]),
     "qr_batch_create": (ctypes.c_int, [ctypes.POINTER(vp), u32, ctypes.POINTER(vp)]),
     "qr_batch_destroy": (ctypes.c_int, [vp]),
     "qr_batch_run": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, vp, vp]),

@@ -491,6 +491,49 @@ int qr_csr_matvec(uint64_t nrows, uint64_t ncols, uint64_t nnz, const uint64_t *
                              (hipStream_t)stream);
 }
 
+int qr_csr_diagonal(uint64_t nrows, uint64_t ncols, uint64_t nnz, const uint64_t *d_indptr,
+                    const uint64_t *d_indices, const double *d_data, double *d_out, void *stream) {
+    if (nrows == 0 || ncols == 0) return QR_OK;
+    if (!d_indptr || !d_out || (nnz && (!d_indices || !d_data)))
+        return fail(QR_ERR_ARG, "null argument");
+    return launch_csr_diagonal(nrows, ncols, nnz, d_indptr, d_indices, d_data, d_out,
+                               (hipStream_t)stream);
+}
+
+int qr_axpby(uint64_t n, const double *a, const double *d_x, const double *b, const double *d_y,
+             double *d_z, void *stream) {
+    if (!a || !b || (n && (!d_x || !d_y || !d_z))) return fail(QR_ERR_ARG, "null argument");
+    return launch_axpby(n, 0, a, d_x, b, d_y, d_z, (hipStream_t)stream);
+}
+
+int qr_axpy(uint64_t n, const double *a, const double *d_x, const double *d_y, double *d_z,
+            void *stream) {
+    if (!a || (n && (!d_x || !d_y || !d_z))) return fail(QR_ERR_ARG, "null argument");
+    return launch_axpby(n, 1, a, d_x, nullptr, d_y, d_z, (hipStream_t)stream);
+}
+
+int qr_ax(uint64_t n, const double *a, const double *d_x, double *d_z, void *stream) {
+    if (!a || (n && (!d_x || !d_z))) return fail(QR_ERR_ARG, "null argument");
+    return launch_axpby(n, 2, a, d_x, nullptr, nullptr, d_z, (hipStream_t)stream);
+}
+
+int qr_precond(uint64_t nrows, uint64_t ncols, uint64_t nnz, const uint64_t *d_indptr,
+               const uint64_t *d_indices, const double *d_data, const double *d_dx,
+               const double *e, double tol, double *d_out, void *stream) {
+    const uint64_t n = nrows < ncols ? nrows : ncols;
+    if (!e || (n && (!d_indptr || !d_dx || !d_out || (nnz && (!d_indices || !d_data)))))
+        return fail(QR_ERR_ARG, "null argument");
+    return launch_precond(n, nullptr, nnz, d_indptr, d_indices, d_data, d_dx, e, tol, d_out,
+                          (hipStream_t)stream);
+}
+
+int qr_precond2(uint64_t n, const double *d_diag, const double *d_dx, const double *e, double tol,
+                double *d_out, void *stream) {
+    if (!e || (n && (!d_diag || !d_dx || !d_out))) return fail(QR_ERR_ARG, "null argument");
+    return launch_precond(n, d_diag, 0, nullptr, nullptr, nullptr, d_dx, e, tol, d_out,
+                          (hipStream_t)stream);
+}
+
 int qr_pauli_apply(const qr_op *op, int32_t precision, uint32_t nvec, const void *d_x, void *d_y,
                    double *d_expect, void *stream) {
     if (!op || !d_x || (!d_y && !d_expect)) return fail(QR_ERR_ARG, "null argument");

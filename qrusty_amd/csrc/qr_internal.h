@@ -133,6 +133,13 @@ int launch_csr_build(const qr_op *op, const DeviceTables *t, uint64_t row_begin,
 int launch_csr_matvec(uint64_t nrows, uint64_t ncols, uint64_t nnz, const uint64_t *d_indptr,
                       const uint64_t *d_indices, const double *d_data, const double *d_x,
                       double *d_y, hipStream_t s);
+int launch_csr_diagonal(uint64_t nrows, uint64_t ncols, uint64_t nnz, const uint64_t *indptr,
+                        const uint64_t *indices, const double *data, double *out, hipStream_t s);
+int launch_axpby(uint64_t n, int mode, const double *a, const double *x, const double *b,
+                 const double *y, double *z, hipStream_t s);
+int launch_precond(uint64_t n, const double *diag, uint64_t nnz, const uint64_t *indptr,
+                   const uint64_t *indices, const double *data, const double *dx, const double *e,
+                   double tol, double *out, hipStream_t s);
 int launch_pauli_apply(const qr_op *op, DeviceTables *t, int32_t precision, uint32_t nvec,
                        const void *d_x, void *d_y, double *d_expect, hipStream_t s);
 

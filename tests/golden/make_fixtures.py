@@ -18,6 +18,9 @@ Outputs (tests/golden/):
   (n <= 12, exact equality as in pyqrusty/tests/test_H.py:32-48).
 * ``config1.json`` / ``config1_y.npy`` -- the CPU-runnable workload (12 qubits,
   200 random terms, seed fixed here) with its CSR digests and y = A x.
+* ``precond_h2.json`` -- dx, e, tol and the printed result vector of the
+  precond tests in pyqrusty/tests/test_it.py:276-300 (inputs and a 9-digit
+  known answer for H2).
 """
 from __future__ import annotations
 
@@ -72,6 +75,18 @@ def parse_rs_fixtures(text: str):
     return out
 
 
+def parse_precond_vectors(text: str):
+    """The dx, e and printed result rv of pyqrusty/tests/test_it.py's precond
+    tests (module-level ``dx=np.array([...])``, ``e=...``, ``rv=np.array([...])``).
+    Complex literals are parsed with complex(); nothing is executed."""
+    def array(name):
+        m = re.search(r"^" + name + r"=np\.array\(\[(.*?)\]", text, re.S | re.M)
+        return [complex(t.strip().replace(" ", "")) for t in m.group(1).split(",") if t.strip()]
+    m = re.search(r"^e=([-+0-9.eE]+)\s*\+\s*0j", text, re.M)
+    tol = float(re.search(r"^tol=([-+0-9.eE]+)", text, re.M).group(1))
+    return {"dx": array("dx"), "e": float(m.group(1)), "rv": array("rv"), "tol": tol}
+
+
 def digest(a: np.ndarray) -> str:
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
@@ -124,6 +139,13 @@ def main():
         print(name, dig[name]["G"], dig[name]["nnz"])
     with open(os.path.join(HERE, "csr_digests.json"), "w") as f:
         json.dump(dig, f, indent=1)
+
+    # the precond inputs and printed result of pyqrusty/tests/test_it.py:276-300
+    with open(os.path.join(args.reference, "pyqrusty/tests/test_it.py")) as f:
+        pc = parse_precond_vectors(f.read())
+    with open(os.path.join(HERE, "precond_h2.json"), "w") as f:
+        json.dump({"dx": [[c.real, c.imag] for c in pc["dx"]], "e": pc["e"], "tol": pc["tol"],
+                   "rv_printed": [[c.real, c.imag] for c in pc["rv"]]}, f, indent=1)
 
     # the CPU-runnable workload: 12 qubits, 200 terms
     labels, coeffs = workloads.random_op(12, 200, seed=workloads.SEED_CFG1)

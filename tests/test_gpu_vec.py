@@ -1,0 +1,165 @@
+"""GPU parity of the eigensolver-step operations: SpMat.diagonal, axpby / axpy
+/ ax and precond / precond2 (pyqrusty/src/lib.rs:118-125, 436-468, 519-591;
+qrusty/src/accel.rs:374-393).
+
+Bar: bit-exact against the oracle (the same num-complex formulas, no fused
+multiply-add on either side).  precond compares |diag - e| with tol through
+hypot, whose last bit may differ between the device and glibc, so the random
+cases keep |diag - e| away from tol; the threshold itself is tested with
+values whose norm is exact.  Also the reference's own tests
+(pyqrusty/tests/test_it.py:46-51, 232-312), rewritten against this package.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import qrusty_amd  # noqa: F401
+    yield
+    torch.cuda.synchronize()
+
+
+def _op(labels, coeffs):
+    import qrusty_amd as qa
+    return qa.SparsePauliOp([qa.Pauli(l) for l in labels], coeffs)
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, dtype=np.complex128).view(np.uint64)
+
+
+@pytest.mark.parametrize("name", ["py_H2", "py_H4", "rs_H6", "py_H8"])
+def test_diagonal_fixtures(hamiltonians, name):
+    labels, coeffs = hamiltonians[name]
+    print(f"[gpu] diagonal {name}", flush=True)
+    m = _op(labels, coeffs).to_matrix_mode("Hip")
+    got = m.diagonal()
+    p = orc.Params(labels, coeffs)
+    ip, ix, dv = orc.build_csr(p)
+    assert np.array_equal(_bits(got), _bits(orc.csr_diagonal((p.N, p.N), ip, ix, dv)))
+    dev = m.diagonal(device=True)
+    assert dev.is_cuda and np.array_equal(_bits(dev.cpu().numpy()), _bits(got))
+
+
+def test_diagonal_general_csr():
+    """Rectangular shapes, missing diagonal entries, empty rows; and the
+    reference's test_spmat (test_it.py:46-51)."""
+    import qrusty_amd as qa
+    import scipy.sparse as sps
+    dmat = np.array([[1.0, 2.0], [3.0, 4.0]], dtype=complex)
+    s = sps.csr_matrix(dmat)
+    m = qa.SpMat.new_unchecked(dmat.shape, s.data, s.indices, s.indptr)
+    assert np.array_equal(m.diagonal(), np.diagonal(dmat))
+    rng = np.random.default_rng(7)
+    for shape in [(3, 2), (2, 5), (64, 64), (100, 37), (1, 1)]:
+        d = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+        d[rng.random(shape) < 0.5] = 0.0
+        d[::3, :] = 0.0  # empty rows
+        s = sps.csr_matrix(d)
+        s.sort_indices()
+        m = qa.SpMat.new_unchecked(shape, s.data, s.indices, s.indptr)
+        assert np.array_equal(m.diagonal(), np.diagonal(d)), shape
+    m.export()
+    with pytest.raises(qa.QrError):
+        m.diagonal()
+
+
+@pytest.mark.parametrize("n", [0, 1, 1000, (1 << 20) + 3])
+def test_axpby_family_bit_exact(n):
+    import qrusty_amd as qa
+    rng = np.random.default_rng(n + 1)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    y = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    a, b = complex(0.3, -1.7), complex(-2.5, 0.25)
+    print(f"[gpu] axpby n={n}", flush=True)
+    assert np.array_equal(_bits(qa.axpby(a, x, b, y)), _bits(orc.axpby(a, x, b, y, 0)))
+    assert np.array_equal(_bits(qa.axpy(a, x, y)), _bits(orc.axpby(a, x, 0, y, 1)))
+    assert np.array_equal(_bits(qa.ax(a, x)), _bits(orc.axpby(a, x, 0, None, 2)))
+    # device tensors stay on the device
+    xt, yt = torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda()
+    zt = qa.axpby(a, xt, b, yt)
+    assert zt.is_cuda and np.array_equal(_bits(zt.cpu().numpy()), _bits(orc.axpby(a, x, b, y, 0)))
+
+
+def test_reference_ax_axpy_axpby():
+    """pyqrusty/tests/test_it.py:232-263."""
+    import qrusty_amd as qa
+    x = np.array([1.0 + 2j, 2.0 + 3j, 3.0 + 4j, 4.0 + 5j])
+    y = np.array([5.0 + 6j, 6.0 + 7j, 7.0 + 8j, 8.0 + 9j])
+    a, b = 0.0 + 1.0j, 2.0 - 1.0j
+    assert np.allclose(qa.ax(a, x), a * x)
+    assert np.allclose(qa.axpy(a, x, y), a * x + y)
+    assert np.allclose(qa.axpby(a, x, b, y), a * x + b * y)
+    rows = 1 << 20
+    rng = np.random.default_rng(0)
+    x = rng.random(rows) + rng.random(rows) * 1j
+    y = rng.random(rows) + rng.random(rows) * 1j
+    assert np.allclose(qa.ax(a, x), a * x)
+    assert np.allclose(qa.axpby(a, x, b, y), a * x + b * y)
+    with pytest.raises(qa.QrError):
+        qa.axpy(a, x, y[:10])
+
+
+def test_precond2_bit_exact_and_threshold():
+    import qrusty_amd as qa
+    rng = np.random.default_rng(11)
+    n = 100000
+    e, tol = complex(0.25, -0.5), 1e-3
+    d = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    far = np.abs(d - e) > 2 * tol  # keep the random cases away from the threshold
+    d = d[far]
+    dx = rng.standard_normal(len(d)) + 1j * rng.standard_normal(len(d))
+    assert np.array_equal(_bits(qa.precond2(d, dx, e, tol)), _bits(orc.precond2(d, dx, e, tol)))
+    # threshold, with exact norms (e = 0, tol = 5 * 2^-12): |v| = 0 and tol/2 are
+    # replaced by tol; |v| = tol (3-4-5 triangle, and -i tol) and 2 tol are not
+    tol = 5.0 / 4096
+    v = np.array([0.0, tol / 2, complex(3.0 / 4096, 4.0 / 4096), 2 * tol, -1j * tol])
+    assert np.array_equal(np.abs(v) < tol, [True, True, False, False, False])
+    dxt = rng.standard_normal(len(v)) + 1j * rng.standard_normal(len(v))
+    got = qa.precond2(v, dxt, 0.0, tol)
+    ref = orc.precond2(v, dxt, 0.0, tol)
+    assert np.array_equal(_bits(got), _bits(ref))
+    assert abs(got[0] - dxt[0] / tol) <= 2e-15 * abs(dxt[0] / tol)
+    assert abs(got[4] - dxt[4] / (-1j * tol)) <= 2e-15 * abs(dxt[4] / tol)
+
+
+def test_precond_from_csr(hamiltonians):
+    """precond reads the diagonal from the CSR: equals precond2 on
+    SpMat.diagonal(), and the reference's test_precond / test_precond2 and
+    printed vector (test_it.py:276-312)."""
+    import qrusty_amd as qa
+    with open(os.path.join(os.path.dirname(__file__), "golden", "precond_h2.json")) as f:
+        pc = json.load(f)
+    dx = np.array([complex(a, b) for a, b in pc["dx"]])
+    rv = np.array([complex(a, b) for a, b in pc["rv_printed"]])
+    labels, coeffs = hamiltonians["py_H2"]
+    m = _op(labels, coeffs).to_matrix()
+    r1 = qa.precond(m, dx, pc["e"], pc["tol"])
+    r2 = qa.precond2(m.diagonal(), dx, pc["e"], pc["tol"])
+    assert np.array_equal(_bits(r1), _bits(r2))
+    diag = m.diagonal()
+
+    def reg(v, tol):
+        return np.where(np.abs(v) < tol, tol, v)
+    assert np.allclose(r1, dx / reg(diag - pc["e"], pc["tol"]))
+    assert np.allclose(r1, rv, rtol=1e-8, atol=1e-20)
+    labels, coeffs = hamiltonians["py_H8"]
+    m = _op(labels, coeffs).to_matrix_mode("Hip")
+    dx = np.random.default_rng(3).standard_normal(1 << 16) + 0j
+    assert np.array_equal(_bits(qa.precond(m, dx, -3.0, 1e-14)),
+                          _bits(qa.precond2(m.diagonal(), dx, -3.0, 1e-14)))
+    m.export()
+    with pytest.raises(qa.QrError):
+        qa.precond(m, dx, -3.0, 1e-14)

@@ -257,3 +257,86 @@ def test_planted_phase_mistakes_are_caught():
     assert not np.array_equal(D, orc.dense_sum(labels, coeffs, "kron"))   # i^b vs (-i)^b
     D2, _, _ = csr_dense(["XY"], [1.0])
     assert not np.array_equal(D2, np.kron(X2, Y2.conj()))               # Y sign convention
+
+
+# ---------------------------------------------------------------------------
+# eigensolver-step operations (pyqrusty/src/lib.rs:118-125, 436-468;
+# qrusty/src/accel.rs:374-393)
+# ---------------------------------------------------------------------------
+def test_diagonal_matches_dense(hamiltonians):
+    """sprs CsMat::diag().to_dense() equals numpy.diagonal of the dense matrix
+    (pyqrusty/tests/test_it.py:46-51), for square fixtures and rectangular /
+    missing-entry cases."""
+    for name in ["py_H2", "py_H4", "rs_H4"]:
+        labels, coeffs = hamiltonians[name]
+        p = orc.Params(labels, coeffs)
+        ip, ix, dv = orc.build_csr(p)
+        dense = orc.dense_sum(labels, coeffs, "rowwise")
+        assert np.array_equal(orc.csr_diagonal((p.N, p.N), ip, ix, dv), np.diagonal(dense))
+    import scipy.sparse as sps
+    rng = np.random.default_rng(4)
+    for shape in [(2, 2), (3, 2), (2, 5), (7, 7)]:
+        d = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+        d[rng.random(shape) < 0.4] = 0.0
+        m = sps.csr_matrix(d)
+        m.sort_indices()
+        got = orc.csr_diagonal(shape, m.indptr.astype(np.uint64), m.indices.astype(np.uint64), m.data)
+        assert np.array_equal(got, np.diagonal(d))
+
+
+def test_axpby_bit_exact_vs_python_complex():
+    """a x + b y, a x + y, a x: the oracle equals CPython's complex * and +
+    (IEEE double, no contraction) bit for bit, and numpy to allclose
+    (pyqrusty/tests/test_it.py:232-263)."""
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(500) + 1j * rng.standard_normal(500)
+    y = rng.standard_normal(500) + 1j * rng.standard_normal(500)
+    x[:3] = [0.0, -0.0 + 0j, 1e-308 + 1e308j]
+    a, b = complex(rng.standard_normal(), rng.standard_normal()), complex(0.0, 1.0)
+    z0 = orc.axpby(a, x, b, y, 0)
+    z1 = orc.axpby(a, x, 0, y, 1)
+    z2 = orc.axpby(a, x, 0, None, 2)
+    for i in range(len(x)):
+        xi, yi = complex(x[i]), complex(y[i])
+        assert z0[i] == a * xi + b * yi or (np.isnan(z0[i]) and np.isnan(a * xi + b * yi))
+        assert z1[i] == a * xi + yi or np.isnan(z1[i])
+        assert z2[i] == a * xi or np.isnan(z2[i])
+    assert np.allclose(z0[3:], a * x[3:] + b * y[3:])
+
+
+def test_precond2_division_and_reg():
+    """dx / reg(diag - e, tol): CPython's complex division (Smith's method) to
+    a few ulp; entries with |diag - e| < tol are replaced by tol + 0i; the
+    vector printed for H2 in pyqrusty/tests/test_it.py:285-300."""
+    rng = np.random.default_rng(6)
+    d = rng.standard_normal(300) + 1j * rng.standard_normal(300)
+    dx = rng.standard_normal(300) + 1j * rng.standard_normal(300)
+    e, tol = 0.3 - 0.1j, 1e-3
+    d[:4] = [e, e + 1e-4, e + 1e-3j, e - 2e-3]  # |v| = 0, < tol, == tol, > tol
+    got = orc.precond2(d, dx, e, tol)
+    for i in range(len(d)):
+        v = complex(d[i]) - e
+        if abs(v) < tol:
+            v = complex(tol, 0.0)
+        ref = complex(dx[i]) / v
+        assert abs(got[i] - ref) <= 2e-15 * abs(ref)
+    # |v| < tol is replaced by tol + 0i; |v| == tol (here v = i tol) is not
+    for i in (0, 1):
+        assert abs(got[i] - dx[i] / tol) <= 2e-15 * abs(dx[i] / tol)
+    assert abs(got[2] - dx[2] / (1e-3j)) <= 2e-15 * abs(dx[2] / 1e-3)
+
+
+def test_precond_h2_printed_vector(hamiltonians):
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "precond_h2.json")) as f:
+        pc = json.load(f)
+    dx = np.array([complex(a, b) for a, b in pc["dx"]])
+    rv = np.array([complex(a, b) for a, b in pc["rv_printed"]])
+    labels, coeffs = hamiltonians["py_H2"]
+    p = orc.Params(labels, coeffs)
+    ip, ix, dv = orc.build_csr(p)
+    diag = orc.csr_diagonal((p.N, p.N), ip, ix, dv)
+    got = orc.precond2(diag, dx, pc["e"], pc["tol"])
+    assert np.allclose(got, rv, rtol=1e-8, atol=1e-20)
+    assert np.allclose(got, dx / (diag - pc["e"]), rtol=1e-15)
