@@ -416,13 +416,35 @@ def bench_vec(args, torch, qa):
     d = torch.empty(N, dtype=torch.complex128, device="cuda")
     t_diag = timed(lambda: L.check(L.lib.qr_csr_diagonal(N, N, dv.numel(), P(ip), P(ix), P(dv),
                                                          P(d), sh)), reps)
-    res = {"workload": "axpby / precond2 on 2^27 complex128; diagonal of the 24-qubit Heisenberg CSR",
+    # H - e*I (a_spmat_p_b_spmat): count pass + indptr scan, then the fill pass
+    eip = torch.arange(N + 1, dtype=torch.int64, device="cuda")
+    eix = torch.arange(N, dtype=torch.int64, device="cuda")
+    edv = torch.ones(N, dtype=torch.complex128, device="cuda")
+    one = (ctypes.c_double * 2)(1.0, 0.0)
+    ipo = torch.empty(N + 1, dtype=torch.int64, device="cuda")
+    nnz = ctypes.c_uint64()
+    common = (N, one, dv.numel(), P(ip), P(ix), P(dv), e, N, P(eip), P(eix), P(edv))
+    L.check(L.lib.qr_csr_add_count(*common, P(ipo), ctypes.byref(nnz), sh))
+    ixo = torch.empty(nnz.value, dtype=torch.int64, device="cuda")
+    dvo = torch.empty(nnz.value, dtype=torch.complex128, device="cuda")
+    t_cnt = timed(lambda: L.check(L.lib.qr_csr_add_count(*common, P(ipo), ctypes.byref(nnz), sh)),
+                  reps)
+    t_fill = timed(lambda: L.check(L.lib.qr_csr_add_fill(*common, P(ipo), P(ixo), P(dvo), sh)), reps)
+    # algorithmic bytes: both inputs (indices + values) and indptrs read once,
+    # the sum written once
+    nin = dv.numel() + N
+    sp_bytes = 24 * nin + 16 * (N + 1) + 24 * nnz.value + 8 * (N + 1)
+    res = {"workload": "axpby / precond2 on 2^27 complex128; diagonal and H - e*I of the "
+                       "24-qubit Heisenberg CSR",
            "axpby_ms": t_axpby, "axpby_GBps": 48 * n / t_axpby / 1e6,
            "precond2_ms": t_pre, "precond2_GBps": 48 * n / t_pre / 1e6,
-           "diagonal_ms": t_diag, "diagonal_rows_per_s": N / (t_diag / 1e3)}
+           "diagonal_ms": t_diag, "diagonal_rows_per_s": N / (t_diag / 1e3),
+           "spadd_count_ms": t_cnt, "spadd_fill_ms": t_fill, "spadd_nnz_in": nin,
+           "spadd_nnz_out": nnz.value, "spadd_GBps": sp_bytes / (t_cnt + t_fill) / 1e6}
     log(f"vec: axpby {t_axpby:.3f} ms ({res['axpby_GBps']:.0f} GB/s), precond2 {t_pre:.3f} ms "
-        f"({res['precond2_GBps']:.0f} GB/s), diagonal {t_diag:.3f} ms ({res['diagonal_rows_per_s']:.3e} rows/s)")
-    del m, ip, ix, dv, d
+        f"({res['precond2_GBps']:.0f} GB/s), diagonal {t_diag:.3f} ms ({res['diagonal_rows_per_s']:.3e} rows/s), "
+        f"H-eI count {t_cnt:.3f} + fill {t_fill:.3f} ms ({res['spadd_GBps']:.0f} GB/s algorithmic)")
+    del m, ip, ix, dv, d, eip, eix, edv, ipo, ixo, dvo
     torch.cuda.empty_cache()
     return res
 

@@ -173,6 +173,24 @@ int qr_precond(uint64_t nrows, uint64_t ncols, uint64_t nnz, const uint64_t *d_i
 int qr_precond2(uint64_t n, const double *d_diag, const double *d_dx, const double *e, double tol,
                 double *d_out, void *stream);
 
+/* a*A + b*B of two CSR matrices of the same shape with ascending columns per
+ * row (pyqrusty's a_spmat_p_b_spmat, pyqrusty/src/lib.rs:499-517: sprs scale
+ * and `+`).  Each output row is the merge of the two rows: a*v1 + b*v2 where
+ * both hold the column, a*v1 + 0 or 0 + b*v2 otherwise; exact zeros are
+ * dropped.  Two calls: qr_csr_add_count writes d_indptr_out (nrows+1) and
+ * returns the output nnz on the host (it synchronises the stream); the caller
+ * allocates, and qr_csr_add_fill writes indices and values. */
+int qr_csr_add_count(uint64_t nrows, const double *a, uint64_t nnz1, const uint64_t *d_indptr1,
+                     const uint64_t *d_indices1, const double *d_data1, const double *b,
+                     uint64_t nnz2, const uint64_t *d_indptr2, const uint64_t *d_indices2,
+                     const double *d_data2, uint64_t *d_indptr_out, uint64_t *nnz_out,
+                     void *stream);
+int qr_csr_add_fill(uint64_t nrows, const double *a, uint64_t nnz1, const uint64_t *d_indptr1,
+                    const uint64_t *d_indices1, const double *d_data1, const double *b,
+                    uint64_t nnz2, const uint64_t *d_indptr2, const uint64_t *d_indices2,
+                    const double *d_data2, const uint64_t *d_indptr_out, uint64_t *d_indices_out,
+                    double *d_data_out, void *stream);
+
 /* ------------------------------------------------------------------------
  * Matrix-free Pauli-sum apply (no counterpart in the reference; equal to the
  * CSR build followed by the CSR matvec):

@@ -78,6 +78,9 @@ def lib():
         L.orc_axpby.restype = None
         L.orc_precond2.argtypes = [ctypes.c_uint64, f64p, f64p, f64p, ctypes.c_double, f64p]
         L.orc_precond2.restype = None
+        L.orc_csr_add.argtypes = [ctypes.c_uint64, f64p, u64p, u64p, f64p, f64p, u64p, u64p, f64p,
+                                  u64p, u64p, f64p]
+        L.orc_csr_add.restype = ctypes.c_uint64
         _lib = L
     return _lib
 
@@ -346,3 +349,24 @@ def precond2(diag, dx, e, tol):
     lib().orc_precond2(len(d), _p(d.view(np.float64), f64p), _p(dxa.view(np.float64), f64p),
                        _p(ev, f64p), float(tol), _p(out.view(np.float64), f64p))
     return out
+
+
+def csr_add(nrows, a, m1, b, m2):
+    """a*A + b*B for CSR triples (indptr, indices, data) with ascending columns
+    (pyqrusty/src/lib.rs:499-517).  Returns (indptr, indices, data)."""
+    def arrs(m):
+        ip, ix, d = m
+        return (np.ascontiguousarray(ip, dtype=np.uint64), np.ascontiguousarray(ix, dtype=np.uint64),
+                _c128(d))
+    ip1, ix1, d1 = arrs(m1)
+    ip2, ix2, d2 = arrs(m2)
+    av = np.array([complex(a).real, complex(a).imag])
+    bv = np.array([complex(b).real, complex(b).imag])
+    ipo = np.empty(nrows + 1, dtype=np.uint64)
+    args = (nrows, _p(av, f64p), _p(ip1, u64p), _p(ix1, u64p), _p(d1.view(np.float64), f64p),
+            _p(bv, f64p), _p(ip2, u64p), _p(ix2, u64p), _p(d2.view(np.float64), f64p))
+    nnz = lib().orc_csr_add(*args, _p(ipo, u64p), None, None)
+    ixo = np.empty(max(nnz, 1), dtype=np.uint64)
+    do = np.empty(max(nnz, 1), dtype=np.complex128)
+    lib().orc_csr_add(*args, _p(ipo, u64p), _p(ixo, u64p), _p(do.view(np.float64), f64p))
+    return ipo, ixo[:nnz], do[:nnz]

@@ -29,6 +29,11 @@
  *                          in test_it.py:285-300 (to its 9 digits).  The bit
  *                          order of num-complex's division is not available
  *                          here: "parity unpinned" at the last bit
+ *   orc_csr_add            scipy's sparse a*A + b*B (which also drops exact
+ *                          zeros) on random and fixture matrices, and the
+ *                          dense sum.  Whether sprs's csmat_binop drops exact
+ *                          zeros is not available here (the crate is absent):
+ *                          "parity unpinned" for that one rule
  */
 #include <math.h>
 #include <stdint.h>
@@ -573,4 +578,59 @@ void orc_precond2(uint64_t n, const double *diag, const double *dx, const double
         out[2 * i] = re / nn;
         out[2 * i + 1] = im / nn;
     }
+}
+
+/* a*A + b*B (pyqrusty/src/lib.rs:499-517; sprs scale and csmat_binop): per row
+ * the merge of the ascending columns; both: a*v1 + b*v2, one side only:
+ * a*v1 + 0 or 0 + b*v2; exact zeros dropped.  With out_indices == NULL only
+ * counts: out_indptr[r+1] - out_indptr[r] is filled and the total returned. */
+uint64_t orc_csr_add(uint64_t nrows, const double *a, const uint64_t *ip1, const uint64_t *ix1,
+                     const double *d1, const double *b, const uint64_t *ip2, const uint64_t *ix2,
+                     const double *d2, uint64_t *out_indptr, uint64_t *out_indices,
+                     double *out_data)
+{
+    uint64_t nnz = 0;
+    out_indptr[0] = 0;
+    for (uint64_t r = 0; r < nrows; ++r) {
+        uint64_t i = ip1[r], i1 = ip1[r + 1], j = ip2[r], j1 = ip2[r + 1];
+        while (i < i1 || j < j1) {
+            uint64_t col;
+            double vr, vi;
+            if (j >= j1 || (i < i1 && ix1[i] < ix2[j])) {
+                col = ix1[i];
+                double pr = a[0] * d1[2 * i] - a[1] * d1[2 * i + 1];
+                double pi = a[0] * d1[2 * i + 1] + a[1] * d1[2 * i];
+                vr = pr + 0.0;
+                vi = pi + 0.0;
+                ++i;
+            } else if (i >= i1 || ix2[j] < ix1[i]) {
+                col = ix2[j];
+                double qr = b[0] * d2[2 * j] - b[1] * d2[2 * j + 1];
+                double qi = b[0] * d2[2 * j + 1] + b[1] * d2[2 * j];
+                vr = 0.0 + qr;
+                vi = 0.0 + qi;
+                ++j;
+            } else {
+                col = ix1[i];
+                double pr = a[0] * d1[2 * i] - a[1] * d1[2 * i + 1];
+                double pi = a[0] * d1[2 * i + 1] + a[1] * d1[2 * i];
+                double qr = b[0] * d2[2 * j] - b[1] * d2[2 * j + 1];
+                double qi = b[0] * d2[2 * j + 1] + b[1] * d2[2 * j];
+                vr = pr + qr;
+                vi = pi + qi;
+                ++i;
+                ++j;
+            }
+            if (vr != 0.0 || vi != 0.0) {
+                if (out_indices) {
+                    out_indices[nnz] = col;
+                    out_data[2 * nnz] = vr;
+                    out_data[2 * nnz + 1] = vi;
+                }
+                ++nnz;
+            }
+        }
+        out_indptr[r + 1] = nnz;
+    }
+    return nnz;
 }

@@ -32,7 +32,7 @@ from . import _lib as L
 from ._lib import QrError
 
 __all__ = ["Pauli", "SparsePauliOp", "SpMat", "spmat_dot_densevec", "csr_matrix", "axpby", "axpy",
-           "ax", "precond", "precond2", "Batch", "QrError"]
+           "ax", "precond", "precond2", "a_spmat_p_b_spmat", "Batch", "QrError"]
 __version__ = "0.1.0"
 
 _PRECISIONS = {"fp64": L.QR_PREC_FP64, "mixed": L.QR_PREC_MIXED}
@@ -379,6 +379,34 @@ class SpMat:
         self._live("cannot access an exported sparse matrix")
         return self._indptr, self._indices, self._data
 
+    def __add__(self, other: "SpMat") -> "SpMat":
+        """self + other (pyqrusty/src/lib.rs:150-152, sprs's `&a + &b`), run as
+        1*self + 1*other: bit-identical for finite entries."""
+        return self._binop(other, "add", 1.0, 1.0)
+
+    def __sub__(self, other: "SpMat") -> "SpMat":
+        """self - other (pyqrusty/src/lib.rs:154-156), run as 1*self + (-1)*other:
+        bit-identical for finite entries."""
+        return self._binop(other, "subtract", 1.0, -1.0)
+
+    def _binop(self, other, name, a, b):
+        if self._data is None or other._data is None:
+            raise QrError(f"cannot {name} already-exported sparse matrix")
+        if self._shape != other._shape:
+            raise QrError(f"cannot {name} matrices with differing shapes "
+                          f"({self._shape} != {other._shape})")
+        return _spadd(a, self, b, other)
+
+    def scale(self, factor) -> None:
+        """In place: every stored value times factor (pyqrusty/src/lib.rs:158-163;
+        num-complex v * factor), through qr_ax on the device."""
+        self._live("cannot scale already-exported sparse matrix")
+        torch = _require_gpu()
+        out = torch.empty_like(self._data)
+        L.check(L.lib.qr_ax(self._data.numel(), _scalar(factor), _ptr(self._data), _ptr(out),
+                            _stream_handle(torch)))
+        self._data = out
+
     def __copy__(self) -> "SpMat":
         if self._data is None:
             m = SpMat.__new__(SpMat)
@@ -512,6 +540,37 @@ def precond2(diag, dx, e, tol: float):
     L.check(L.lib.qr_precond2(dt.numel(), _ptr(dt), _ptr(dxt), _scalar(e), float(tol), _ptr(z),
                               _stream_handle(torch)))
     return _vec_out(torch, z, np_in)
+
+
+def a_spmat_p_b_spmat(a, spmat1: SpMat, b, spmat2: SpMat) -> SpMat:
+    """a*spmat1 + b*spmat2 on the device (pyqrusty/src/lib.rs:499-517); rows
+    must hold ascending columns; exact zeros are dropped."""
+    if spmat1._data is None:
+        raise QrError("cannot a*m1+b*m2 with an exported sparse matrix (first argument)")
+    if spmat2._data is None:
+        raise QrError("cannot a*m1+b*m2 with an exported sparse matrix (second argument)")
+    if spmat1._shape != spmat2._shape:
+        raise QrError(f"cannot a*m1+b*m2 with matrices with differing shapes "
+                      f"({spmat1._shape} != {spmat2._shape})")
+    return _spadd(a, spmat1, b, spmat2)
+
+
+def _spadd(a, spmat1: SpMat, b, spmat2: SpMat) -> SpMat:
+    """Count pass, indptr scan and fill pass of qr_csr_add_count / _fill."""
+    torch = _require_gpu()
+    rows, cols = spmat1._shape
+    dev = spmat1._data.device
+    indptr = torch.empty(rows + 1, dtype=torch.int64, device=dev)
+    nnz = ctypes.c_uint64()
+    common = (rows, _scalar(a), spmat1._data.numel(), _ptr(spmat1._indptr), _ptr(spmat1._indices),
+              _ptr(spmat1._data), _scalar(b), spmat2._data.numel(), _ptr(spmat2._indptr),
+              _ptr(spmat2._indices), _ptr(spmat2._data))
+    L.check(L.lib.qr_csr_add_count(*common, _ptr(indptr), ctypes.byref(nnz), _stream_handle(torch)))
+    indices = torch.empty(nnz.value, dtype=torch.int64, device=dev)
+    data = torch.empty(nnz.value, dtype=torch.complex128, device=dev)
+    L.check(L.lib.qr_csr_add_fill(*common, _ptr(indptr), _ptr(indices), _ptr(data),
+                                  _stream_handle(torch)))
+    return SpMat._from_device((rows, cols), indptr, indices, data)
 
 
 def csr_matrix(m: SpMat):

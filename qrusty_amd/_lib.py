@@ -48,6 +48,10 @@ SIGNATURES = {
     "qr_ax": (ctypes.c_int, [u64, f64p, vp, vp, vp]),
     "qr_precond": (ctypes.c_int, [u64, u64, u64, vp, vp, vp, vp, f64p, ctypes.c_double, vp, vp]),
     "qr_precond2": (ctypes.c_int, [u64, vp, vp, f64p, ctypes.c_double, vp, vp]),
+    "qr_csr_add_count": (ctypes.c_int, [u64, f64p, u64, vp, vp, vp, f64p, u64, vp, vp, vp, vp,
+                                        u64p, vp]),
+    "qr_csr_add_fill": (ctypes.c_int, [u64, f64p, u64, vp, vp, vp, f64p, u64, vp, vp, vp, vp, vp,
+                                       vp, vp]),
     "qr_batch_create": (ctypes.c_int, [ctypes.POINTER(vp), u32, ctypes.POINTER(vp)]),
     "qr_batch_destroy": (ctypes.c_int, [vp]),
     "qr_batch_run": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, vp, vp]),

@@ -534,6 +534,39 @@ int qr_precond2(uint64_t n, const double *d_diag, const double *d_dx, const doub
                           (hipStream_t)stream);
 }
 
+int qr_csr_add_count(uint64_t nrows, const double *a, uint64_t nnz1, const uint64_t *d_indptr1,
+                     const uint64_t *d_indices1, const double *d_data1, const double *b,
+                     uint64_t nnz2, const uint64_t *d_indptr2, const uint64_t *d_indices2,
+                     const double *d_data2, uint64_t *d_indptr_out, uint64_t *nnz_out,
+                     void *stream) {
+    if (!a || !b || !nnz_out || !d_indptr_out) return fail(QR_ERR_ARG, "null argument");
+    *nnz_out = 0;
+    if (nrows == 0) {
+        QR_HIP_CHECK(hipMemsetAsync(d_indptr_out, 0, sizeof(uint64_t), (hipStream_t)stream));
+        return QR_OK;
+    }
+    if (!d_indptr1 || !d_indptr2 || (nnz1 && (!d_indices1 || !d_data1)) ||
+        (nnz2 && (!d_indices2 || !d_data2)))
+        return fail(QR_ERR_ARG, "null argument");
+    return launch_spadd_count(nrows, a, nnz1, d_indptr1, d_indices1, d_data1, b, nnz2, d_indptr2,
+                              d_indices2, d_data2, d_indptr_out, nnz_out, (hipStream_t)stream);
+}
+
+int qr_csr_add_fill(uint64_t nrows, const double *a, uint64_t nnz1, const uint64_t *d_indptr1,
+                    const uint64_t *d_indices1, const double *d_data1, const double *b,
+                    uint64_t nnz2, const uint64_t *d_indptr2, const uint64_t *d_indices2,
+                    const double *d_data2, const uint64_t *d_indptr_out, uint64_t *d_indices_out,
+                    double *d_data_out, void *stream) {
+    if (!a || !b || !d_indptr_out) return fail(QR_ERR_ARG, "null argument");
+    if (nrows == 0) return QR_OK;
+    if (!d_indptr1 || !d_indptr2 || (nnz1 && (!d_indices1 || !d_data1)) ||
+        (nnz2 && (!d_indices2 || !d_data2)))
+        return fail(QR_ERR_ARG, "null argument");
+    return launch_spadd_fill(nrows, a, nnz1, d_indptr1, d_indices1, d_data1, b, nnz2, d_indptr2,
+                             d_indices2, d_data2, d_indptr_out, d_indices_out, d_data_out,
+                             (hipStream_t)stream);
+}
+
 int qr_pauli_apply(const qr_op *op, int32_t precision, uint32_t nvec, const void *d_x, void *d_y,
                    double *d_expect, void *stream) {
     if (!op || !d_x || (!d_y && !d_expect)) return fail(QR_ERR_ARG, "null argument");

@@ -140,6 +140,14 @@ int launch_axpby(uint64_t n, int mode, const double *a, const double *x, const d
 int launch_precond(uint64_t n, const double *diag, uint64_t nnz, const uint64_t *indptr,
                    const uint64_t *indices, const double *data, const double *dx, const double *e,
                    double tol, double *out, hipStream_t s);
+int launch_spadd_count(uint64_t nrows, const double *a, uint64_t nnz1, const uint64_t *ip1,
+                       const uint64_t *ix1, const double *d1, const double *b, uint64_t nnz2,
+                       const uint64_t *ip2, const uint64_t *ix2, const double *d2, uint64_t *ipo,
+                       uint64_t *nnz_out, hipStream_t s);
+int launch_spadd_fill(uint64_t nrows, const double *a, uint64_t nnz1, const uint64_t *ip1,
+                      const uint64_t *ix1, const double *d1, const double *b, uint64_t nnz2,
+                      const uint64_t *ip2, const uint64_t *ix2, const double *d2,
+                      const uint64_t *ipo, uint64_t *ixo, double *dout, hipStream_t s);
 int launch_pauli_apply(const qr_op *op, DeviceTables *t, int32_t precision, uint32_t nvec,
                        const void *d_x, void *d_y, double *d_expect, hipStream_t s);
 

@@ -163,3 +163,97 @@ def test_precond_from_csr(hamiltonians):
     m.export()
     with pytest.raises(qa.QrError):
         qa.precond(m, dx, -3.0, 1e-14)
+
+
+def _sp(qa, d):
+    import scipy.sparse as sps
+    s = sps.csr_matrix(d)
+    s.sort_indices()
+    return s, qa.SpMat.new_unchecked(d.shape, s.data, s.indices, s.indptr)
+
+
+def _host(m):
+    _, data, indices, indptr = m.__copy__().export()
+    return indptr, indices, data
+
+
+def test_spadd_bit_exact_random():
+    """a*A + b*B against the oracle's merge: indptr and indices equal, values
+    to the last bit; exact cancellations vanish; rectangular shapes, empty
+    rows, one side empty."""
+    import qrusty_amd as qa
+    rng = np.random.default_rng(21)
+    for shape, dens in [((200, 200), 0.05), ((37, 300), 0.2), ((1, 1), 1.0), ((64, 5), 0.0),
+                        ((5000, 5000), 0.002)]:
+        d1 = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+        d2 = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+        d1[rng.random(shape) >= dens] = 0.0
+        d2[rng.random(shape) >= 2 * dens] = 0.0
+        d1[::4, :] = 0.0
+        k = min(shape) // 2
+        d2[:k, :k] = -d1[:k, :k]  # cancel exactly for a = b = 1
+        s1, m1 = _sp(qa, d1)
+        s2, m2 = _sp(qa, d2)
+        for a, b in [(1.0, 1.0), (0.3 - 1.1j, -2.0 + 0.5j), (0.0, 1j)]:
+            print(f"[gpu] spadd shape={shape} a={a} b={b}", flush=True)
+            got = _host(qa.a_spmat_p_b_spmat(a, m1, b, m2))
+            ref = orc.csr_add(shape[0], a, (s1.indptr, s1.indices, s1.data), b,
+                              (s2.indptr, s2.indices, s2.data))
+            assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+            assert np.array_equal(_bits(got[2]), _bits(ref[2]))
+
+
+def test_spadd_pauli_shift(hamiltonians):
+    """H - e*I on a Pauli CSR (the Davidson shift): bit-exact vs the oracle and
+    equal to the dense a*H + b*I; SpMat + and - and scale."""
+    import qrusty_amd as qa
+    labels, coeffs = hamiltonians["py_H8"]
+    h = _op(labels, coeffs).to_matrix_mode("Hip")
+    n = h.shape()[0]
+    eye = qa.SparsePauliOp([qa.Pauli("I" * len(labels[0]))], [1.0]).to_matrix()
+    e = -3.25 + 0.125j
+    print("[gpu] spadd H8 shift", flush=True)
+    got = _host(qa.a_spmat_p_b_spmat(1.0, h, -e, eye))
+    p = orc.Params(labels, coeffs)
+    ref = orc.csr_add(n, 1.0, orc.build_csr(p), -e,
+                      (np.arange(n + 1, dtype=np.uint64), np.arange(n, dtype=np.uint64),
+                       np.ones(n, complex)))
+    for g_, r_ in zip(got, ref):
+        assert np.array_equal(np.asarray(g_).view(np.uint64), np.asarray(r_).view(np.uint64))
+    hs = _host(h)
+    s = _host(h + eye)
+    d = _host(h - eye)
+    r_add = orc.csr_add(n, 1.0, hs, 1.0, _host(eye))
+    r_sub = orc.csr_add(n, 1.0, hs, -1.0, _host(eye))
+    for g_, r_ in list(zip(s, r_add)) + list(zip(d, r_sub)):
+        assert np.array_equal(np.asarray(g_).view(np.uint64), np.asarray(r_).view(np.uint64))
+    z = _host(h - h)
+    assert z[1].size == 0 and np.array_equal(z[0], np.zeros(n + 1, dtype=np.uint64))
+    hc = h.__copy__()
+    hc.scale(0.5 - 2j)
+    assert np.array_equal(_bits(_host(hc)[2]), _bits(orc.axpby(0.5 - 2j, hs[2], 0, None, 2)))
+
+
+def test_reference_add_interpolate():
+    """pyqrusty/tests/test_it.py:103-139."""
+    import qrusty_amd as qa
+    import scipy.sparse as sps  # noqa: F401
+    I = np.array([[1, 0], [0, 1]], dtype=complex)
+    X = np.array([[0, 1], [1, 0]], dtype=complex)
+    matI = qa.Pauli("I").to_matrix()
+    matX = qa.Pauli("X").to_matrix()
+    assert np.array_equal(qa.csr_matrix(matI + matX).todense(), I + X)
+    a, b = complex(1.0), complex(1.0j)
+    assert np.array_equal(qa.csr_matrix(qa.a_spmat_p_b_spmat(a, matI, b, matX)).todense(),
+                          a * I + b * X)
+    with pytest.raises(Exception):
+        qa.a_spmat_p_b_spmat(a, matI, b, qa.Pauli("XI").to_matrix())
+    assert np.array_equal(qa.csr_matrix(matI - matI).todense(), I - I)
+    gone = qa.Pauli("X").to_matrix()
+    gone.export()
+    with pytest.raises(qa.QrError, match="first argument"):
+        qa.a_spmat_p_b_spmat(a, gone, b, matX)
+    with pytest.raises(qa.QrError, match="second argument"):
+        qa.a_spmat_p_b_spmat(a, matX, b, gone)
+    with pytest.raises(qa.QrError):
+        gone + matX

@@ -340,3 +340,38 @@ def test_precond_h2_printed_vector(hamiltonians):
     got = orc.precond2(diag, dx, pc["e"], pc["tol"])
     assert np.allclose(got, rv, rtol=1e-8, atol=1e-20)
     assert np.allclose(got, dx / (diag - pc["e"]), rtol=1e-15)
+
+
+def test_csr_add_vs_scipy_and_dense(hamiltonians):
+    """a*A + b*B: the oracle's merge equals scipy's sparse sum (which also drops
+    exact zeros) in structure, and its values to the last bit where scipy's
+    products round the same way; the dense sum within 1e-15."""
+    import scipy.sparse as sps
+    rng = np.random.default_rng(8)
+    for shape, dens in [((50, 50), 0.2), ((30, 70), 0.1), ((1, 1), 1.0)]:
+        d1 = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+        d2 = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+        d1[rng.random(shape) > dens] = 0.0
+        d2[rng.random(shape) > dens] = 0.0
+        d2[:5, :5] = -d1[:5, :5]  # a = b = 1: these cancel exactly and must vanish
+        for a, b in [(1.0, 1.0), (0.5 - 2j, 1j), (0.0, 3.0)]:
+            m1, m2 = sps.csr_matrix(d1), sps.csr_matrix(d2)
+            ip, ix, dv = orc.csr_add(shape[0], a, (m1.indptr, m1.indices, m1.data), b,
+                                     (m2.indptr, m2.indices, m2.data))
+            ref = (a * m1 + b * m2).tocsr()
+            ref.sort_indices()
+            ref.eliminate_zeros()
+            assert np.array_equal(ip, ref.indptr) and np.array_equal(ix, ref.indices)
+            assert np.allclose(dv, ref.data, rtol=1e-15, atol=0)
+            got = sps.csr_matrix((dv, ix.astype(np.int64), ip.astype(np.int64)), shape=shape).toarray()
+            assert np.allclose(got, a * d1 + b * d2, rtol=1e-15, atol=1e-300)
+    # two Pauli CSRs of different structure: H2 and the identity shifted
+    labels, coeffs = hamiltonians["py_H2"]
+    p = orc.Params(labels, coeffs)
+    m1 = orc.build_csr(p)
+    n = p.N
+    eye = (np.arange(n + 1, dtype=np.uint64), np.arange(n, dtype=np.uint64), np.ones(n, complex))
+    ip, ix, dv = orc.csr_add(n, 1.0, m1, -0.7, eye)
+    dense = orc.dense_sum(labels, coeffs, "rowwise") - 0.7 * np.eye(n)
+    got = sps.csr_matrix((dv, ix.astype(np.int64), ip.astype(np.int64)), shape=(n, n)).toarray()
+    assert np.allclose(got, dense, rtol=1e-15, atol=1e-300)
