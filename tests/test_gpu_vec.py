@@ -183,13 +183,20 @@ def test_spadd_bit_exact_random():
     rows, one side empty."""
     import qrusty_amd as qa
     rng = np.random.default_rng(21)
+    # the last two: rows longer than a staged tile (the global-memory merge),
+    # and mostly short rows with a few long ones (both routes in one launch)
     for shape, dens in [((200, 200), 0.05), ((37, 300), 0.2), ((1, 1), 1.0), ((64, 5), 0.0),
-                        ((5000, 5000), 0.002)]:
+                        ((5000, 5000), 0.002), ((6, 3000), 0.3), ((3000, 2000), "uneven")]:
+        if dens == "uneven":
+            dens = np.where(np.arange(shape[0])[:, None] % 97 == 5, 0.5, 0.002)
         d1 = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
         d2 = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
         d1[rng.random(shape) >= dens] = 0.0
-        d2[rng.random(shape) >= 2 * dens] = 0.0
+        d2[rng.random(shape) >= np.minimum(2 * np.asarray(dens), 1.0)] = 0.0
         d1[::4, :] = 0.0
+        # -0.0 components: one-sided entries get + 0, which makes them +0.0
+        d1[1::4, :] = np.where(d1[1::4, :] != 0, np.abs(d1[1::4, :].real) * -1 - 0.0j, 0)
+        d1[1::4, :].imag[d1[1::4, :] != 0] = -0.0
         k = min(shape) // 2
         d2[:k, :k] = -d1[:k, :k]  # cancel exactly for a = b = 1
         s1, m1 = _sp(qa, d1)
