@@ -191,6 +191,19 @@ int qr_csr_add_fill(uint64_t nrows, const double *a, uint64_t nnz1, const uint64
                     const double *d_data2, const uint64_t *d_indptr_out, uint64_t *d_indices_out,
                     double *d_data_out, void *stream);
 
+/* Tolerance filters (qrusty/src/util.rs:144-171; SpMat.count_zeros and
+ * SpMat.eliminate_zeros, pyqrusty/src/lib.rs:170-183).  |v| = hypot(re, im).
+ *  qr_csr_count_zeros     : *count = #entries with |v| <= tol (synchronises).
+ *  qr_csr_eliminate_zeros : the CSR of the entries with |v| > tol, in their
+ *                           order; outputs sized for nnz entries (the upper
+ *                           bound), *nnz_out = kept entries (synchronises). */
+int qr_csr_count_zeros(uint64_t nnz, const double *d_data, double tol, uint64_t *count,
+                       void *stream);
+int qr_csr_eliminate_zeros(uint64_t nrows, uint64_t nnz, const uint64_t *d_indptr,
+                           const uint64_t *d_indices, const double *d_data, double tol,
+                           uint64_t *d_indptr_out, uint64_t *d_indices_out, double *d_data_out,
+                           uint64_t *nnz_out, void *stream);
+
 /* ------------------------------------------------------------------------
  * Matrix-free Pauli-sum apply (no counterpart in the reference; equal to the
  * CSR build followed by the CSR matvec):

@@ -81,6 +81,11 @@ def lib():
         L.orc_csr_add.argtypes = [ctypes.c_uint64, f64p, u64p, u64p, f64p, f64p, u64p, u64p, f64p,
                                   u64p, u64p, f64p]
         L.orc_csr_add.restype = ctypes.c_uint64
+        L.orc_count_zeros.argtypes = [ctypes.c_uint64, f64p, ctypes.c_double]
+        L.orc_count_zeros.restype = ctypes.c_uint64
+        L.orc_eliminate_zeros.argtypes = [ctypes.c_uint64, u64p, u64p, f64p, ctypes.c_double, u64p,
+                                          u64p, f64p]
+        L.orc_eliminate_zeros.restype = ctypes.c_uint64
         _lib = L
     return _lib
 
@@ -370,3 +375,23 @@ def csr_add(nrows, a, m1, b, m2):
     do = np.empty(max(nnz, 1), dtype=np.complex128)
     lib().orc_csr_add(*args, _p(ipo, u64p), _p(ixo, u64p), _p(do.view(np.float64), f64p))
     return ipo, ixo[:nnz], do[:nnz]
+
+
+def count_zeros(data, tol):
+    """Entries with hypot(re, im) <= tol (qrusty/src/util.rs:144-152)."""
+    d = _c128(data)
+    return int(lib().orc_count_zeros(len(d), _p(d.view(np.float64), f64p), tol))
+
+
+def eliminate_zeros(nrows, m, tol):
+    """CSR triple without the entries with hypot(re, im) <= tol
+    (qrusty/src/util.rs:154-171)."""
+    ip = np.ascontiguousarray(m[0], dtype=np.uint64)
+    ix = np.ascontiguousarray(m[1], dtype=np.uint64)
+    d = _c128(m[2])
+    ipo = np.empty(nrows + 1, dtype=np.uint64)
+    ixo = np.empty(max(len(d), 1), dtype=np.uint64)
+    do = np.empty(max(len(d), 1), dtype=np.complex128)
+    k = lib().orc_eliminate_zeros(nrows, _p(ip, u64p), _p(ix, u64p), _p(d.view(np.float64), f64p), tol,
+                                  _p(ipo, u64p), _p(ixo, u64p), _p(do.view(np.float64), f64p))
+    return ipo, ixo[:k], do[:k]

@@ -29,6 +29,9 @@
  *                          in test_it.py:285-300 (to its 9 digits).  The bit
  *                          order of num-complex's division is not available
  *                          here: "parity unpinned" at the last bit
+ *   orc_count_zeros,       numpy's abs (hypot) on random data, the
+ *   orc_eliminate_zeros    reference's test_nz (test_it.py:140-147) and
+ *                          scipy's CSR of the kept entries
  *   orc_csr_add            scipy's sparse a*A + b*B (which also drops exact
  *                          zeros) on random and fixture matrices, and the
  *                          dense sum.  Whether sprs's csmat_binop drops exact
@@ -633,4 +636,35 @@ uint64_t orc_csr_add(uint64_t nrows, const double *a, const uint64_t *ip1, const
         out_indptr[r + 1] = nnz;
     }
     return nnz;
+}
+
+/* count_zeros / eliminate_zeros (qrusty/src/util.rs:144-171): |v| is
+ * num-complex's norm(), hypot(re, im).  eliminate writes the kept entries in
+ * order (rows of a CSR with ascending columns stay as they are) and returns
+ * their number; out_indptr has nrows + 1 entries. */
+uint64_t orc_count_zeros(uint64_t nnz, const double *d, double tol)
+{
+    uint64_t n = 0;
+    for (uint64_t i = 0; i < nnz; ++i) n += hypot(d[2 * i], d[2 * i + 1]) <= tol;
+    return n;
+}
+
+uint64_t orc_eliminate_zeros(uint64_t nrows, const uint64_t *ip, const uint64_t *ix, const double *d,
+                             double tol, uint64_t *out_indptr, uint64_t *out_indices,
+                             double *out_data)
+{
+    uint64_t k = 0;
+    out_indptr[0] = 0;
+    for (uint64_t r = 0; r < nrows; ++r) {
+        for (uint64_t i = ip[r]; i < ip[r + 1]; ++i) {
+            if (hypot(d[2 * i], d[2 * i + 1]) > tol) {
+                out_indices[k] = ix[i];
+                out_data[2 * k] = d[2 * i];
+                out_data[2 * k + 1] = d[2 * i + 1];
+                ++k;
+            }
+        }
+        out_indptr[r + 1] = k;
+    }
+    return k;
 }

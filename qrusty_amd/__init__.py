@@ -390,11 +390,11 @@ class SpMat:
         return self._binop(other, "subtract", 1.0, -1.0)
 
     def _binop(self, other, name, a, b):
+        """pyqrusty/src/lib.rs:40-52 (SpMat::binop), same messages."""
         if self._data is None or other._data is None:
-            raise QrError(f"cannot {name} already-exported sparse matrix")
+            raise QrError(f"cannot {name} already-exported sparse matrices")
         if self._shape != other._shape:
-            raise QrError(f"cannot {name} matrices with differing shapes "
-                          f"({self._shape} != {other._shape})")
+            raise QrError(f"cannot {name} matrices with different shapes")
         return _spadd(a, self, b, other)
 
     def scale(self, factor) -> None:
@@ -406,6 +406,35 @@ class SpMat:
         L.check(L.lib.qr_ax(self._data.numel(), _scalar(factor), _ptr(self._data), _ptr(out),
                             _stream_handle(torch)))
         self._data = out
+
+    def count_zeros(self, tolerance: float = 1e-7) -> int:
+        """Entries with |v| <= tolerance (pyqrusty/src/lib.rs:170-175,
+        qrusty/src/util.rs:144-152), counted on the device."""
+        self._live("cannot count zeroes of an exported sparse matrix")
+        torch = _require_gpu()
+        n = ctypes.c_uint64()
+        L.check(L.lib.qr_csr_count_zeros(self._data.numel(), _ptr(self._data), float(tolerance),
+                                         ctypes.byref(n), _stream_handle(torch)))
+        return int(n.value)
+
+    def eliminate_zeros(self, tolerance: float = 1e-7) -> "SpMat":
+        """A new SpMat without the entries with |v| <= tolerance
+        (pyqrusty/src/lib.rs:177-183, qrusty/src/util.rs:154-171), compacted
+        on the device.  Rows keep their column order (the reference's triplet
+        rebuild sorts columns, which leaves sorted rows unchanged)."""
+        self._live("cannot eliminate zeroes of an exported sparse matrix")
+        torch = _require_gpu()
+        nnz = self._data.numel()
+        ipo = torch.empty_like(self._indptr)
+        ixo = torch.empty(max(nnz, 1), dtype=torch.int64, device=self._data.device)
+        dvo = torch.empty(max(nnz, 1), dtype=torch.complex128, device=self._data.device)
+        kept = ctypes.c_uint64()
+        L.check(L.lib.qr_csr_eliminate_zeros(self._shape[0], nnz, _ptr(self._indptr),
+                                             _ptr(self._indices), _ptr(self._data),
+                                             float(tolerance), _ptr(ipo), _ptr(ixo), _ptr(dvo),
+                                             ctypes.byref(kept), _stream_handle(torch)))
+        k = int(kept.value)
+        return SpMat._from_device(self._shape, ipo, ixo[:k], dvo[:k])
 
     def __copy__(self) -> "SpMat":
         if self._data is None:

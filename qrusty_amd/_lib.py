@@ -25,6 +25,7 @@ u64p = ctypes.POINTER(ctypes.c_uint64)
 u32p = ctypes.POINTER(ctypes.c_uint32)
 i32p = ctypes.POINTER(ctypes.c_int32)
 f64p = ctypes.POINTER(ctypes.c_double)
+f64 = ctypes.c_double
 
 # name -> (restype, argtypes); mirrors include/qrhip.h
 SIGNATURES = {
@@ -52,6 +53,8 @@ SIGNATURES = {
                                         u64p, vp]),
     "qr_csr_add_fill": (ctypes.c_int, [u64, f64p, u64, vp, vp, vp, f64p, u64, vp, vp, vp, vp, vp,
                                        vp, vp]),
+    "qr_csr_count_zeros": (ctypes.c_int, [u64, vp, f64, u64p, vp]),
+    "qr_csr_eliminate_zeros": (ctypes.c_int, [u64, u64, vp, vp, vp, f64, vp, vp, vp, u64p, vp]),
     "qr_batch_create": (ctypes.c_int, [ctypes.POINTER(vp), u32, ctypes.POINTER(vp)]),
     "qr_batch_destroy": (ctypes.c_int, [vp]),
     "qr_batch_run": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, vp, vp]),

@@ -567,6 +567,24 @@ int qr_csr_add_fill(uint64_t nrows, const double *a, uint64_t nnz1, const uint64
                              (hipStream_t)stream);
 }
 
+int qr_csr_count_zeros(uint64_t nnz, const double *d_data, double tol, uint64_t *count,
+                       void *stream) {
+    if (!count || (nnz && !d_data)) return fail(QR_ERR_ARG, "null argument");
+    return launch_count_zeros(nnz, d_data, tol, count, (hipStream_t)stream);
+}
+
+int qr_csr_eliminate_zeros(uint64_t nrows, uint64_t nnz, const uint64_t *d_indptr,
+                           const uint64_t *d_indices, const double *d_data, double tol,
+                           uint64_t *d_indptr_out, uint64_t *d_indices_out, double *d_data_out,
+                           uint64_t *nnz_out, void *stream) {
+    if (!nnz_out || !d_indptr || !d_indptr_out ||
+        (nnz && (!d_indices || !d_data || !d_indices_out || !d_data_out)))
+        return fail(QR_ERR_ARG, "null argument");
+    *nnz_out = 0;
+    return launch_eliminate_zeros(nrows, nnz, d_indptr, d_indices, d_data, tol, d_indptr_out,
+                                  d_indices_out, d_data_out, nnz_out, (hipStream_t)stream);
+}
+
 int qr_pauli_apply(const qr_op *op, int32_t precision, uint32_t nvec, const void *d_x, void *d_y,
                    double *d_expect, void *stream) {
     if (!op || !d_x || (!d_y && !d_expect)) return fail(QR_ERR_ARG, "null argument");

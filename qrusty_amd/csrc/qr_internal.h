@@ -148,6 +148,10 @@ int launch_spadd_fill(uint64_t nrows, const double *a, uint64_t nnz1, const uint
                       const uint64_t *ix1, const double *d1, const double *b, uint64_t nnz2,
                       const uint64_t *ip2, const uint64_t *ix2, const double *d2,
                       const uint64_t *ipo, uint64_t *ixo, double *dout, hipStream_t s);
+int launch_count_zeros(uint64_t nnz, const double *d, double tol, uint64_t *count, hipStream_t s);
+int launch_eliminate_zeros(uint64_t nrows, uint64_t nnz, const uint64_t *ip, const uint64_t *ix,
+                           const double *d, double tol, uint64_t *ipo, uint64_t *ixo, double *dout,
+                           uint64_t *nnz_out, hipStream_t s);
 int launch_pauli_apply(const qr_op *op, DeviceTables *t, int32_t precision, uint32_t nvec,
                        const void *d_x, void *d_y, double *d_expect, hipStream_t s);
 

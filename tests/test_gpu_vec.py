@@ -264,3 +264,47 @@ def test_reference_add_interpolate():
         qa.a_spmat_p_b_spmat(a, matX, b, gone)
     with pytest.raises(qa.QrError):
         gone + matX
+
+
+def test_reference_nz():
+    """pyqrusty/tests/test_it.py:140-147."""
+    import qrusty_amd as qa
+    matI = qa.Pauli("I").to_matrix()
+    matI.scale(1e-8 + 0j)
+    assert matI.count_zeros() == 2
+    mat2 = matI.eliminate_zeros()
+    assert mat2.count_zeros() == 0
+    assert mat2.nnz() == 0
+    gone = qa.Pauli("I").to_matrix()
+    gone.export()
+    with pytest.raises(qa.QrError, match="count zeroes"):
+        gone.count_zeros()
+    with pytest.raises(qa.QrError, match="eliminate zeroes"):
+        gone.eliminate_zeros()
+
+
+@pytest.mark.parametrize("tol", [0.0, 0.05, 0.5, 5.0])
+def test_zero_filters_vs_oracle(tol):
+    """count_zeros / eliminate_zeros bit-exact vs the oracle on a random CSR
+    (magnitudes kept away from tol, where device and host hypot could round
+    apart) and on |3 + 4i| = 5 exactly at the threshold."""
+    import qrusty_amd as qa
+    import scipy.sparse as sps
+    rng = np.random.default_rng(int(tol * 100) + 3)
+    m = sps.random(3000, 700, density=0.05, random_state=9, format="csr", dtype=np.float64)
+    mag = m.data * 2
+    mag = np.where(np.abs(mag - tol) < 1e-6, mag + 1e-3, mag)
+    m = sps.csr_matrix((mag * np.exp(2j * np.pi * rng.random(m.nnz)), m.indices, m.indptr),
+                       shape=m.shape)
+    m.sort_indices()
+    m.data[::11] = 3 + 4j
+    sm = qa.SpMat.new_unchecked(m.shape, m.data, m.indices, m.indptr)
+    print(f"[gpu] zero filters tol={tol}", flush=True)
+    assert sm.count_zeros(tol) == orc.count_zeros(m.data, tol)
+    got = _host(sm.eliminate_zeros(tol))
+    ref = orc.eliminate_zeros(m.shape[0], (m.indptr, m.indices, m.data), tol)
+    assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+    assert np.array_equal(_bits(got[2]), _bits(ref[2]))
+    five = qa.SpMat.new_unchecked((1, 1), [3 + 4j], [0], [0, 1])
+    assert five.count_zeros(5.0) == 1 and five.count_zeros(np.nextafter(5.0, 0)) == 0
+    assert five.eliminate_zeros(5.0).nnz() == 0 and five.eliminate_zeros(4.99).nnz() == 1

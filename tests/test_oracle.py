@@ -375,3 +375,32 @@ def test_csr_add_vs_scipy_and_dense(hamiltonians):
     dense = orc.dense_sum(labels, coeffs, "rowwise") - 0.7 * np.eye(n)
     got = sps.csr_matrix((dv, ix.astype(np.int64), ip.astype(np.int64)), shape=(n, n)).toarray()
     assert np.allclose(got, dense, rtol=1e-15, atol=1e-300)
+
+
+def test_count_and_eliminate_zeros():
+    """The reference's test_nz (pyqrusty/tests/test_it.py:140-147): I scaled by
+    1e-8 has 2 entries at or below 1e-7 and none after elimination; random
+    CSRs against numpy's abs (hypot) and scipy's CSR of the kept entries;
+    |v| == tol exactly counts as zero."""
+    import scipy.sparse as sps
+    d = np.array([1e-8 + 0j, 1e-8 + 0j])
+    assert orc.count_zeros(d, 1e-7) == 2
+    ip, ix, dv = orc.eliminate_zeros(2, (np.array([0, 1, 2]), np.array([0, 1]), d), 1e-7)
+    assert len(dv) == 0 and np.array_equal(ip, [0, 0, 0])
+    rng = np.random.default_rng(4)
+    m = sps.random(300, 200, density=0.1, random_state=5, format="csr", dtype=np.float64)
+    m = sps.csr_matrix((m.data * np.exp(2j * np.pi * rng.random(m.nnz)), m.indices, m.indptr),
+                       shape=m.shape)
+    m.sort_indices()
+    m.data[::7] = 0.3 * np.exp(1j * 0.7)   # exactly on the threshold for some tol
+    for tol in [0.0, 0.1, 0.3, 0.5, 2.0]:
+        assert orc.count_zeros(m.data, tol) == int(np.sum(np.abs(m.data) <= tol))
+        ip, ix, dv = orc.eliminate_zeros(300, (m.indptr, m.indices, m.data), tol)
+        keep = np.abs(m.data) > tol
+        ref = sps.csr_matrix((np.where(keep, m.data, 0), m.indices, m.indptr), shape=m.shape)
+        ref.eliminate_zeros()
+        assert np.array_equal(ip, ref.indptr) and np.array_equal(ix, ref.indices)
+        assert np.array_equal(dv, ref.data)
+    # |3 + 4i| = 5 exactly
+    assert orc.count_zeros(np.array([3 + 4j]), 5.0) == 1
+    assert orc.count_zeros(np.array([3 + 4j]), np.nextafter(5.0, 0)) == 0
