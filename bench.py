@@ -23,6 +23,7 @@ import argparse
 import ctypes
 import json
 import os
+import shutil
 import statistics
 import sys
 import time
@@ -416,6 +417,14 @@ def bench_vec(args, torch, qa):
     d = torch.empty(N, dtype=torch.complex128, device="cuda")
     t_diag = timed(lambda: L.check(L.lib.qr_csr_diagonal(N, N, dv.numel(), P(ip), P(ix), P(dv),
                                                          P(d), sh)), reps)
+    # CSR matvec of the same matrix (one complex128 state): the "24-qubit
+    # Heisenberg matvec" in CSR form, next to the matrix-free apply
+    xs = torch.randn(N, dtype=torch.complex128, device="cuda", generator=g)
+    ys = torch.empty_like(xs)
+    t_mv = timed(lambda: L.check(L.lib.qr_csr_matvec(N, N, dv.numel(), P(ip), P(ix), P(dv), P(xs),
+                                                     P(ys), sh)), reps)
+    mv_bytes = 24 * dv.numel() + 8 * (N + 1) + 32 * N
+    del xs, ys
     # H - e*I (a_spmat_p_b_spmat): count pass + indptr scan, then the fill pass
     eip = torch.arange(N + 1, dtype=torch.int64, device="cuda")
     eix = torch.arange(N, dtype=torch.int64, device="cuda")
@@ -439,13 +448,70 @@ def bench_vec(args, torch, qa):
            "axpby_ms": t_axpby, "axpby_GBps": 48 * n / t_axpby / 1e6,
            "precond2_ms": t_pre, "precond2_GBps": 48 * n / t_pre / 1e6,
            "diagonal_ms": t_diag, "diagonal_rows_per_s": N / (t_diag / 1e3),
+           "heisenberg24_csr_matvec_ms": t_mv, "heisenberg24_csr_matvec_GBps": mv_bytes / t_mv / 1e6,
+           "heisenberg24_csr_matvec_frac": mv_bytes / t_mv / 1e6 / HBM_PEAK_GBPS,
            "spadd_count_ms": t_cnt, "spadd_fill_ms": t_fill, "spadd_nnz_in": nin,
            "spadd_nnz_out": nnz.value, "spadd_GBps": sp_bytes / (t_cnt + t_fill) / 1e6}
     log(f"vec: axpby {t_axpby:.3f} ms ({res['axpby_GBps']:.0f} GB/s), precond2 {t_pre:.3f} ms "
         f"({res['precond2_GBps']:.0f} GB/s), diagonal {t_diag:.3f} ms ({res['diagonal_rows_per_s']:.3e} rows/s), "
-        f"H-eI count {t_cnt:.3f} + fill {t_fill:.3f} ms ({res['spadd_GBps']:.0f} GB/s algorithmic)")
+        f"H-eI count {t_cnt:.3f} + fill {t_fill:.3f} ms ({res['spadd_GBps']:.0f} GB/s algorithmic), "
+        f"Heisenberg-24 CSR matvec {t_mv:.3f} ms ({res['heisenberg24_csr_matvec_GBps']:.0f} GB/s, "
+        f"{100 * res['heisenberg24_csr_matvec_frac']:.1f}% of HBM peak)")
     del m, ip, ix, dv, d, eip, eix, edv, ipo, ixo, dvo
     torch.cuda.empty_cache()
+    return res
+
+
+def bench_io(args, torch, qa):
+    """CSR files of the 20-qubit Heisenberg ring CSR (2^20 rows, G = 21, 2.2e7
+    entries) in the temporary directory: raw format write / read (24 B per
+    entry + indptr) and MatrixMarket write / read (text), plain; raw with
+    gzip.  Times include the device <-> host streaming; files are removed."""
+    import tempfile
+    from qrusty_amd import workloads
+    labels, coeffs = workloads.heisenberg_ring(20, seed=workloads.SEED_CFG3)
+    m = qa.SparsePauliOp([qa.Pauli(l) for l in labels], coeffs).to_matrix_mode("Hip")
+    nnz = m.nnz()
+    raw_bytes = 24 * nnz + 8 * ((1 << 20) + 1) + 2 + 6 * 8
+    res = {"workload": "CSR files of the 20-qubit Heisenberg ring (2^20 rows, 2.2e7 entries)",
+           "nnz": nnz}
+    d = tempfile.mkdtemp()
+
+    def clock(fn):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t, out
+
+    try:
+        for name, path in [("raw", os.path.join(d, "h.bin")), ("raw_gz", os.path.join(d, "h.bin.gz"))]:
+            tw, _ = clock(lambda: m.raw_write(path))
+            tr, back = clock(lambda: qa.SpMat.raw_read(path))
+            assert all(torch.equal(a.view(torch.int64), b.view(torch.int64))
+                       for a, b in zip(m.device_arrays(), back.device_arrays()))
+            res[f"{name}_write_s"], res[f"{name}_read_s"] = tw, tr
+            res[f"{name}_file_bytes"] = os.path.getsize(path)
+            res[f"{name}_write_GBps"] = raw_bytes / tw / 1e9
+            res[f"{name}_read_GBps"] = raw_bytes / tr / 1e9
+            os.remove(path)
+            del back
+        path = os.path.join(d, "h.mtx")
+        tw, _ = clock(lambda: m.matrixmarket_write(path))
+        size = os.path.getsize(path)
+        tr, back = clock(lambda: qa.SpMat.matrixmarket_read(path))
+        assert all(torch.equal(a.view(torch.int64), b.view(torch.int64))
+                   for a, b in zip(m.device_arrays(), back.device_arrays()))
+        os.remove(path)
+        res.update({"mm_write_s": tw, "mm_read_s": tr, "mm_file_bytes": size,
+                    "mm_write_entries_per_s": nnz / tw, "mm_read_entries_per_s": nnz / tr,
+                    "mm_write_MBps": size / tw / 1e6, "mm_read_MBps": size / tr / 1e6})
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    log(f"io: raw write {res['raw_write_GBps']:.2f} GB/s read {res['raw_read_GBps']:.2f} GB/s; "
+        f"raw.gz write {res['raw_gz_write_GBps']:.2f} read {res['raw_gz_read_GBps']:.2f} GB/s; "
+        f"mtx write {res['mm_write_MBps']:.0f} MB/s ({res['mm_write_entries_per_s']:.3e} entries/s) "
+        f"read {res['mm_read_MBps']:.0f} MB/s")
     return res
 
 
@@ -479,7 +545,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the Heisenberg apply workload")
     ap.add_argument("--no-check", action="store_true", help="skip the sampled parity check")
-    ap.add_argument("--only", choices=["csr", "apply", "batch", "large", "vec", "copy"], default=None)
+    ap.add_argument("--only", choices=["csr", "apply", "batch", "large", "vec", "io", "copy"],
+                    default=None, help="one section; io (files) runs only when named")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
@@ -524,6 +591,8 @@ def main():
         extra["large_apply"] = bench_large_apply(args, torch, qa)
     if args.only in (None, "vec") and not args.no_extra:
         extra["vec"] = bench_vec(args, torch, qa)
+    if args.only == "io":
+        extra["io"] = bench_io(args, torch, qa)
     out["extra"] = extra
     print(json.dumps(out), flush=True)
 

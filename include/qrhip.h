@@ -38,7 +38,9 @@ enum qr_status {
     QR_ERR_MODE = -4,     /* unrecognized AccelMode string (lib.rs:327-329)     */
     QR_ERR_SIZE = -5,     /* problem too large for the requested operation      */
     QR_ERR_HIP = -6,      /* a HIP runtime call failed                          */
-    QR_ERR_NOMEM = -7     /* allocation failed                                  */
+    QR_ERR_NOMEM = -7,    /* allocation failed                                  */
+    QR_ERR_IO = -8,       /* a file could not be opened, read or written        */
+    QR_ERR_FORMAT = -9    /* a file's content is malformed                      */
 };
 
 /* Message of the last failure on this thread ("" if none). */
@@ -203,6 +205,53 @@ int qr_csr_eliminate_zeros(uint64_t nrows, uint64_t nnz, const uint64_t *d_indpt
                            const uint64_t *d_indices, const double *d_data, double tol,
                            uint64_t *d_indptr_out, uint64_t *d_indices_out, double *d_data_out,
                            uint64_t *nnz_out, void *stream);
+
+/* ------------------------------------------------------------------------
+ * CSR files of a device matrix.  A path whose extension is "gz" is gzip
+ * (qrusty/src/util.rs:316-369).  Entries stream between the device and the
+ * file through pinned buffers; the calls synchronise the stream.
+ *
+ * Raw binary format of qrusty::rawio::write / read (qrusty/src/rawio.rs:
+ * 128-179): "MI", u64 storage (0 = CSR), rows, cols, then indptr, indices and
+ * data each preceded by its u64 length, in the writer's byte order (a file
+ * written with the other byte order is swapped on reading).
+ *  qr_csr_write_raw     : writes a CSR of rows + 1 indptr words.
+ *  qr_csr_read_raw_info : rows, cols and nnz of a file (CSR storage only).
+ *  qr_csr_read_raw      : reads it into caller buffers of rows + 1, nnz and
+ *                         2 * nnz words (rows and nnz as the info call gave).
+ *
+ * MatrixMarket coordinate complex (SpMat.matrixmarket_write / _read /
+ * _write_symmetric, pyqrusty/src/lib.rs:216-259; sprs io 0.11): header
+ * "%%MatrixMarket matrix coordinate complex <symmetry>", "% written by sprs",
+ * "rows cols count", then "row col re im" per written entry (1-based; the
+ * floats as Rust's Display prints them).  QR_MM_GENERAL writes every entry;
+ * the symmetric modes the lower triangle (row >= col; row > col for
+ * skew-symmetric).  Reading accepts complex, real and integer fields and
+ * every symmetry, expands the symmetric ones (mirror v, -v or conj v), sums
+ * duplicates in file order and sorts the columns of each row.
+ *  qr_mm_read_open  : parses a file into a reader holding the device CSR and
+ *                     reports its shape and nnz.
+ *  qr_mm_read_take  : copies the CSR into caller buffers (rows + 1, nnz,
+ *                     2 * nnz words).
+ *  qr_mm_read_close : frees the reader.
+ * ---------------------------------------------------------------------- */
+enum { QR_MM_GENERAL = 0, QR_MM_SYMMETRIC = 1, QR_MM_HERMITIAN = 2, QR_MM_SKEW_SYMMETRIC = 3 };
+typedef struct qr_mm_reader qr_mm_reader;
+
+int qr_csr_write_raw(const char *path, uint64_t rows, uint64_t cols, uint64_t nnz,
+                     const uint64_t *d_indptr, const uint64_t *d_indices, const double *d_data,
+                     void *stream);
+int qr_csr_read_raw_info(const char *path, uint64_t *rows, uint64_t *cols, uint64_t *nnz);
+int qr_csr_read_raw(const char *path, uint64_t rows, uint64_t nnz, uint64_t *d_indptr,
+                    uint64_t *d_indices, double *d_data, void *stream);
+int qr_csr_write_mm(const char *path, uint64_t rows, uint64_t cols, uint64_t nnz,
+                    const uint64_t *d_indptr, const uint64_t *d_indices, const double *d_data,
+                    int32_t symmetry, void *stream);
+int qr_mm_read_open(const char *path, qr_mm_reader **out, uint64_t *rows, uint64_t *cols,
+                    uint64_t *nnz, void *stream);
+int qr_mm_read_take(const qr_mm_reader *reader, uint64_t *d_indptr, uint64_t *d_indices,
+                    double *d_data, void *stream);
+int qr_mm_read_close(qr_mm_reader *reader);
 
 /* ------------------------------------------------------------------------
  * Matrix-free Pauli-sum apply (no counterpart in the reference; equal to the

@@ -16,12 +16,23 @@ Two layers:
   (lib.rs:93-114, 203-213, 401-412) that uses ``scipy.sparse.kron`` of the 2x2
   Pauli matrices -- a textbook routine independent of the x/z bit tricks.
 
+* restatements of the CSR file formats (pure Python): ``rust_display_f64``
+  (Rust's Display for f64), ``mm_text`` (qrusty/src/util.rs:172-199) and
+  ``raw_bytes`` (qrusty/src/rawio.rs:128-147).
+
 Parity status: every function is pinned by tests/test_oracle.py (known-answer
 tests from lib.rs:608-813 and pyqrusty/tests/test_it.py, dense Kronecker
 equality on the H2/H4/H6 fixtures as in pyqrusty/tests/test_H.py:32-48,
 brute force on random small operators).  The bit order of the CSR matvec sum
 (inside the absent third-party crate sprs) is "parity unpinned" at the bit
-level and pinned to tolerance only.
+level and pinned to tolerance only.  The file formats: ``rust_display_f64``
+is pinned by Rust's documented outputs, the reference's repr test ("1" for
+1.0) and a round-trip plus shortest-digits check; ``mm_text`` by scipy's
+mmread reading it back exactly; ``raw_bytes`` by an independent decode at
+the documented offsets and the reference's rawio test matrix.  sprs's own
+symmetric writer (write_matrix_market_sym; the crate is absent) is restated
+as the MatrixMarket format stores a symmetric matrix, the lower triangle:
+"parity unpinned" beyond that.
 """
 from __future__ import annotations
 
@@ -395,3 +406,73 @@ def eliminate_zeros(nrows, m, tol):
     k = lib().orc_eliminate_zeros(nrows, _p(ip, u64p), _p(ix, u64p), _p(d.view(np.float64), f64p), tol,
                                   _p(ipo, u64p), _p(ixo, u64p), _p(do.view(np.float64), f64p))
     return ipo, ixo[:k], do[:k]
+
+
+# ---------------------------------------------------------------------------
+# CSR files
+# ---------------------------------------------------------------------------
+def rust_display_f64(v: float) -> str:
+    """Rust's Display for f64: the shortest digits that read back to v
+    (Python's repr has the same digits), written positionally, never with an
+    exponent; "NaN", "inf", "-inf"; -0.0 gives "-0"."""
+    if v != v:
+        return "NaN"
+    if v in (float("inf"), float("-inf")):
+        return "inf" if v > 0 else "-inf"
+    sign = "-" if str(v).startswith("-") else ""
+    m, _, e = ("%r" % abs(v)).partition("e")
+    digits = m.replace(".", "")
+    point = (m.index(".") if "." in m else len(m)) + (int(e) if e else 0)
+    digits = digits.lstrip("0")
+    point -= len(m.replace(".", "")) - len(m.replace(".", "").lstrip("0"))
+    digits = digits.rstrip("0")
+    if not digits:
+        return sign + "0"
+    if point <= 0:
+        return sign + "0." + "0" * (-point) + digits
+    if point >= len(digits):
+        return sign + digits + "0" * (point - len(digits))
+    return sign + digits[:point] + "." + digits[point:]
+
+
+_MM_SYM = {"general": 0, "symmetric": 1, "hermitian": 2, "skew-symmetric": 3}
+
+
+def mm_text(shape, indptr, indices, data, symmetry="general") -> str:
+    """MatrixMarket text as qrusty's util::write_matrix_market_for_scipy
+    writes it (qrusty/src/util.rs:172-199): header, "% written by sprs",
+    "rows cols nnz", then "row col re im" 1-based per entry in row order.  A
+    symmetric mode names itself in the header and keeps the lower triangle
+    (row >= col; row > col for skew-symmetric), as the MatrixMarket format
+    stores symmetric matrices (sprs's write_matrix_market_sym; the crate is
+    not in the reference tree)."""
+    sym = _MM_SYM[symmetry]
+    rows, cols = shape
+    lines = []
+    for r in range(rows):
+        for e in range(int(indptr[r]), int(indptr[r + 1])):
+            c = int(indices[e])
+            if sym != 0 and (c > r or (sym == 3 and c == r)):
+                continue
+            v = complex(data[e])
+            lines.append(f"{r + 1} {c + 1} {rust_display_f64(v.real)} {rust_display_f64(v.imag)}\n")
+    return (f"%%MatrixMarket matrix coordinate complex {symmetry}\n% written by sprs\n"
+            f"{rows} {cols} {len(lines)}\n" + "".join(lines))
+
+
+def raw_bytes(shape, indptr, indices, data, byteorder="<") -> bytes:
+    """qrusty::rawio::write (qrusty/src/rawio.rs:128-147): the bytes "MI" as
+    the writer's u16 (so "IM" when read in the other order), then u64
+    storage 0 (CSR), rows, cols, and the three arrays each after its u64
+    length, all in the writer's byte order (`byteorder` "<" or ">")."""
+    import struct
+    mark = b"MI" if byteorder == "<" else b"IM"
+    u = np.dtype(np.uint64).newbyteorder(byteorder)
+    f = np.dtype(np.float64).newbyteorder(byteorder)
+    d = np.ascontiguousarray(data, dtype=np.complex128).view(np.float64)
+    out = [mark, struct.pack(byteorder + "4Q", 0, shape[0], shape[1], len(indptr)),
+           np.asarray(indptr, dtype=np.uint64).astype(u).tobytes(),
+           struct.pack(byteorder + "Q", len(indices)),
+           np.asarray(indices, dtype=np.uint64).astype(u).tobytes(),
+           struct.pack(byteorder + "Q", len(d) // 2), d.astype(f).tobytes()]
+    return b"".join(out)

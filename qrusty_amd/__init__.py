@@ -29,10 +29,11 @@ import math
 import numpy as np
 
 from . import _lib as L
-from ._lib import QrError
+from ._lib import QrError, QrIOError
 
 __all__ = ["Pauli", "SparsePauliOp", "SpMat", "spmat_dot_densevec", "csr_matrix", "axpby", "axpy",
-           "ax", "precond", "precond2", "a_spmat_p_b_spmat", "Batch", "QrError"]
+           "ax", "precond", "precond2", "a_spmat_p_b_spmat", "Batch", "QrError",
+           "QrIOError"]
 __version__ = "0.1.0"
 
 _PRECISIONS = {"fp64": L.QR_PREC_FP64, "mixed": L.QR_PREC_MIXED}
@@ -329,8 +330,20 @@ class SpMat:
 
     @staticmethod
     def new_unchecked(shape, data, indices, indptr) -> "SpMat":
-        """Wraps CSR arrays without validation (pyqrusty/src/lib.rs:103-116)."""
+        """Wraps CSR arrays without validation (pyqrusty/src/lib.rs:103-116).
+        Host arrays are copied to the device; torch device tensors (complex128
+        data, 64-bit integer indices and indptr, contiguous) are adopted as
+        they are, without a copy."""
         torch = _require_gpu()
+        if all(isinstance(a, torch.Tensor) and a.is_cuda for a in (data, indices, indptr)):
+            ok = (data.dtype == torch.complex128 and data.is_contiguous() and
+                  all(a.dtype in (torch.int64, torch.uint64) and a.is_contiguous()
+                      for a in (indices, indptr)))
+            if not ok:
+                raise QrError("SpMat.new_unchecked: device tensors must be contiguous complex128 "
+                              "data and 64-bit integer indices and indptr")
+            return SpMat._from_device(tuple(shape), indptr.view(torch.int64),
+                                      indices.view(torch.int64), data)
         d = np.ascontiguousarray(np.asarray(data, dtype=np.complex128))
         i = np.ascontiguousarray(np.asarray(indices).astype(np.uint64)).view(np.int64)
         p = np.ascontiguousarray(np.asarray(indptr).astype(np.uint64)).view(np.int64)
@@ -435,6 +448,75 @@ class SpMat:
                                              ctypes.byref(kept), _stream_handle(torch)))
         k = int(kept.value)
         return SpMat._from_device(self._shape, ipo, ixo[:k], dvo[:k])
+
+    # -- files ----------------------------------------------------------------
+    def matrixmarket_write(self, save_path: str) -> None:
+        """MatrixMarket coordinate complex general file ("*.gz": gzip);
+        pyqrusty/src/lib.rs:216-229 (sprs io, util.rs:172-199)."""
+        self._live("cannot write an already-destroyed sparse matrix")
+        self._write_mm(save_path, 0)
+
+    def matrixmarket_write_symmetric(self, save_path: str, mode: str = "general") -> None:
+        """The same with a symmetry mode ("general", "symmetric", "hermitian",
+        "skew-symmetric"): the lower triangle is written (pyqrusty/src/lib.rs:
+        88-99, 248-259).  The matrix is taken to have that symmetry."""
+        if mode not in L.QR_MM_SYMMETRY:
+            raise QrError(f"matrixmarket_write_symmetric: unrecognized mode {mode}")
+        self._live("cannot write an already-destroyed sparse matrix")
+        self._write_mm(save_path, L.QR_MM_SYMMETRY[mode])
+
+    def _write_mm(self, path, sym):
+        torch = _require_gpu()
+        L.check(L.lib.qr_csr_write_mm(str(path).encode(), self._shape[0], self._shape[1],
+                                      self._data.numel(), _ptr(self._indptr), _ptr(self._indices),
+                                      _ptr(self._data), sym, _stream_handle(torch)))
+
+    @staticmethod
+    def matrixmarket_read(read_path: str) -> "SpMat":
+        """Reads a MatrixMarket coordinate file (any symmetry, complex / real /
+        integer field; "*.gz": gzip) into a device CSR with sorted rows and
+        summed duplicates (pyqrusty/src/lib.rs:231-246)."""
+        torch = _require_gpu()
+        rd = ctypes.c_void_p()
+        rows, cols, nnz = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        L.check(L.lib.qr_mm_read_open(str(read_path).encode(), ctypes.byref(rd), ctypes.byref(rows),
+                                      ctypes.byref(cols), ctypes.byref(nnz), _stream_handle(torch)))
+        try:
+            dev = torch.device("cuda", torch.cuda.current_device())
+            indptr = torch.empty(rows.value + 1, dtype=torch.int64, device=dev)
+            indices = torch.empty(nnz.value, dtype=torch.int64, device=dev)
+            data = torch.empty(nnz.value, dtype=torch.complex128, device=dev)
+            L.check(L.lib.qr_mm_read_take(rd, _ptr(indptr), _ptr(indices), _ptr(data),
+                                          _stream_handle(torch)))
+        finally:
+            L.lib.qr_mm_read_close(rd)
+        return SpMat._from_device((rows.value, cols.value), indptr, indices, data)
+
+    def raw_write(self, path: str) -> None:
+        """qrusty's raw binary CSR format (qrusty/src/rawio.rs:128-147);
+        "*.gz": gzip."""
+        self._live("cannot write an already-destroyed sparse matrix")
+        torch = _require_gpu()
+        L.check(L.lib.qr_csr_write_raw(str(path).encode(), self._shape[0], self._shape[1],
+                                       self._data.numel(), _ptr(self._indptr), _ptr(self._indices),
+                                       _ptr(self._data), _stream_handle(torch)))
+
+    @staticmethod
+    def raw_read(path: str) -> "SpMat":
+        """Reads qrusty's raw binary CSR format, either byte order
+        (qrusty/src/rawio.rs:149-179)."""
+        torch = _require_gpu()
+        rows, cols, nnz = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        p = str(path).encode()
+        L.check(L.lib.qr_csr_read_raw_info(p, ctypes.byref(rows), ctypes.byref(cols),
+                                           ctypes.byref(nnz)))
+        dev = torch.device("cuda", torch.cuda.current_device())
+        indptr = torch.empty(rows.value + 1, dtype=torch.int64, device=dev)
+        indices = torch.empty(nnz.value, dtype=torch.int64, device=dev)
+        data = torch.empty(nnz.value, dtype=torch.complex128, device=dev)
+        L.check(L.lib.qr_csr_read_raw(p, rows.value, nnz.value, _ptr(indptr), _ptr(indices),
+                                      _ptr(data), _stream_handle(torch)))
+        return SpMat._from_device((rows.value, cols.value), indptr, indices, data)
 
     def __copy__(self) -> "SpMat":
         if self._data is None:

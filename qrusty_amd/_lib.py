@@ -55,6 +55,13 @@ SIGNATURES = {
                                        vp, vp]),
     "qr_csr_count_zeros": (ctypes.c_int, [u64, vp, f64, u64p, vp]),
     "qr_csr_eliminate_zeros": (ctypes.c_int, [u64, u64, vp, vp, vp, f64, vp, vp, vp, u64p, vp]),
+    "qr_csr_write_raw": (ctypes.c_int, [ctypes.c_char_p, u64, u64, u64, vp, vp, vp, vp]),
+    "qr_csr_read_raw_info": (ctypes.c_int, [ctypes.c_char_p, u64p, u64p, u64p]),
+    "qr_csr_read_raw": (ctypes.c_int, [ctypes.c_char_p, u64, u64, vp, vp, vp, vp]),
+    "qr_csr_write_mm": (ctypes.c_int, [ctypes.c_char_p, u64, u64, u64, vp, vp, vp, i32, vp]),
+    "qr_mm_read_open": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(vp), u64p, u64p, u64p, vp]),
+    "qr_mm_read_take": (ctypes.c_int, [vp, vp, vp, vp, vp]),
+    "qr_mm_read_close": (ctypes.c_int, [vp]),
     "qr_batch_create": (ctypes.c_int, [ctypes.POINTER(vp), u32, ctypes.POINTER(vp)]),
     "qr_batch_destroy": (ctypes.c_int, [vp]),
     "qr_batch_run": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, vp, vp]),
@@ -96,6 +103,16 @@ def _load():
 lib = _load()
 
 
+class QrIOError(QrError, OSError):
+    """A file could not be opened, read or written (the reference raises
+    OSError for these, pyqrusty/src/lib.rs:216-259)."""
+
+
+QR_ERR_IO = -8
+QR_MM_SYMMETRY = {"": 0, "general": 0, "symmetric": 1, "hermitian": 2, "skew-symmetric": 3}
+
+
 def check(rc: int) -> None:
     if rc != QR_OK:
-        raise QrError(lib.qr_last_error().decode(errors="replace"), rc)
+        msg = lib.qr_last_error().decode(errors="replace")
+        raise (QrIOError if rc == QR_ERR_IO else QrError)(msg, rc)

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libqrhip.so")
 SOURCES = ["qr_api.cpp", "qr_csr.hip", "qr_apply.hip", "qr_batch.hip", "qr_vec.hip",
-           "qr_spadd.hip", "qr_filter.hip"]
+           "qr_spadd.hip", "qr_filter.hip", "qr_io.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -27,10 +27,14 @@ FLAGS = [
 ]
 
 
+# zlib for gzip-compressed CSR files
+LIBS = ["-lz"]
+
+
 def build(verbose: bool = False) -> str:
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIB + ".tmp"
-    cmd = [HIPCC] + FLAGS + ["-o", tmp] + srcs
+    cmd = [HIPCC] + FLAGS + ["-o", tmp] + srcs + LIBS
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)

@@ -585,6 +585,62 @@ int qr_csr_eliminate_zeros(uint64_t nrows, uint64_t nnz, const uint64_t *d_indpt
                                   d_indices_out, d_data_out, nnz_out, (hipStream_t)stream);
 }
 
+int qr_csr_write_raw(const char *path, uint64_t rows, uint64_t cols, uint64_t nnz,
+                     const uint64_t *d_indptr, const uint64_t *d_indices, const double *d_data,
+                     void *stream) {
+    if (!path || !d_indptr || (nnz && (!d_indices || !d_data))) return fail(QR_ERR_ARG, "null argument");
+    return io_write_raw(path, rows, cols, nnz, d_indptr, d_indices, d_data, (hipStream_t)stream);
+}
+
+int qr_csr_read_raw_info(const char *path, uint64_t *rows, uint64_t *cols, uint64_t *nnz) {
+    if (!path || !rows || !cols || !nnz) return fail(QR_ERR_ARG, "null argument");
+    return io_read_raw_info(path, rows, cols, nnz);
+}
+
+int qr_csr_read_raw(const char *path, uint64_t rows, uint64_t nnz, uint64_t *d_indptr,
+                    uint64_t *d_indices, double *d_data, void *stream) {
+    if (!path || !d_indptr || (nnz && (!d_indices || !d_data))) return fail(QR_ERR_ARG, "null argument");
+    return io_read_raw(path, rows, nnz, d_indptr, d_indices, d_data, (hipStream_t)stream);
+}
+
+int qr_csr_write_mm(const char *path, uint64_t rows, uint64_t cols, uint64_t nnz,
+                    const uint64_t *d_indptr, const uint64_t *d_indices, const double *d_data,
+                    int32_t symmetry, void *stream) {
+    if (!path || !d_indptr || (nnz && (!d_indices || !d_data))) return fail(QR_ERR_ARG, "null argument");
+    return io_write_mm(path, rows, cols, nnz, d_indptr, d_indices, d_data, symmetry,
+                       (hipStream_t)stream);
+}
+
+int qr_mm_read_open(const char *path, qr_mm_reader **out, uint64_t *rows, uint64_t *cols,
+                    uint64_t *nnz, void *stream) {
+    if (!path || !out || !rows || !cols || !nnz) return fail(QR_ERR_ARG, "null argument");
+    *out = nullptr;
+    const int rc = io_read_mm(path, out, (hipStream_t)stream);
+    if (rc != QR_OK) return rc;
+    *rows = (*out)->rows;
+    *cols = (*out)->cols;
+    *nnz = (*out)->nnz;
+    return QR_OK;
+}
+
+int qr_mm_read_take(const qr_mm_reader *r, uint64_t *d_indptr, uint64_t *d_indices, double *d_data,
+                    void *stream) {
+    if (!r || !d_indptr || (r->nnz && (!d_indices || !d_data))) return fail(QR_ERR_ARG, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    QR_HIP_CHECK(hipMemcpyAsync(d_indptr, r->indptr, (r->rows + 1) * 8, hipMemcpyDeviceToDevice, s));
+    if (r->nnz) {
+        QR_HIP_CHECK(hipMemcpyAsync(d_indices, r->indices, r->nnz * 8, hipMemcpyDeviceToDevice, s));
+        QR_HIP_CHECK(hipMemcpyAsync(d_data, r->data, r->nnz * 16, hipMemcpyDeviceToDevice, s));
+    }
+    QR_HIP_CHECK(hipStreamSynchronize(s));
+    return QR_OK;
+}
+
+int qr_mm_read_close(qr_mm_reader *r) {
+    delete r;
+    return QR_OK;
+}
+
 int qr_pauli_apply(const qr_op *op, int32_t precision, uint32_t nvec, const void *d_x, void *d_y,
                    double *d_expect, void *stream) {
     if (!op || !d_x || (!d_y && !d_expect)) return fail(QR_ERR_ARG, "null argument");

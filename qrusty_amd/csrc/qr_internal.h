@@ -12,6 +12,18 @@
 
 struct qr_op;
 
+// a parsed MatrixMarket file: the CSR in device memory (qr_mm_read_open)
+struct qr_mm_reader {
+    uint64_t rows = 0, cols = 0, nnz = 0;
+    uint64_t *indptr = nullptr, *indices = nullptr;  // device
+    double *data = nullptr;                          // device, re/im pairs
+    ~qr_mm_reader() {
+        if (indptr) (void)hipFree(indptr);
+        if (indices) (void)hipFree(indices);
+        if (data) (void)hipFree(data);
+    }
+};
+
 namespace qr {
 
 // Sets the thread's last error and returns `code`.
@@ -152,6 +164,14 @@ int launch_count_zeros(uint64_t nnz, const double *d, double tol, uint64_t *coun
 int launch_eliminate_zeros(uint64_t nrows, uint64_t nnz, const uint64_t *ip, const uint64_t *ix,
                            const double *d, double tol, uint64_t *ipo, uint64_t *ixo, double *dout,
                            uint64_t *nnz_out, hipStream_t s);
+int io_write_raw(const char *path, uint64_t rows, uint64_t cols, uint64_t nnz, const uint64_t *ip,
+                 const uint64_t *ix, const double *d, hipStream_t s);
+int io_read_raw_info(const char *path, uint64_t *rows, uint64_t *cols, uint64_t *nnz);
+int io_read_raw(const char *path, uint64_t rows, uint64_t nnz, uint64_t *ip, uint64_t *ix, double *d,
+                hipStream_t s);
+int io_write_mm(const char *path, uint64_t rows, uint64_t cols, uint64_t nnz, const uint64_t *ip,
+                const uint64_t *ix, const double *d, int sym, hipStream_t s);
+int io_read_mm(const char *path, qr_mm_reader **out, hipStream_t s);
 int launch_pauli_apply(const qr_op *op, DeviceTables *t, int32_t precision, uint32_t nvec,
                        const void *d_x, void *d_y, double *d_expect, hipStream_t s);
 

@@ -308,3 +308,18 @@ def test_zero_filters_vs_oracle(tol):
     five = qa.SpMat.new_unchecked((1, 1), [3 + 4j], [0], [0, 1])
     assert five.count_zeros(5.0) == 1 and five.count_zeros(np.nextafter(5.0, 0)) == 0
     assert five.eliminate_zeros(5.0).nnz() == 0 and five.eliminate_zeros(4.99).nnz() == 1
+
+
+def test_device_handoff_zero_copy():
+    """Device tensors in and out without a copy: new_unchecked adopts torch
+    tensors, device_arrays returns the same storage, DLPack round trip."""
+    import qrusty_amd as qa
+    from torch.utils import dlpack
+    m = qa.SparsePauliOp([qa.Pauli("XZ"), qa.Pauli("YY")], [1.0, 0.5j]).to_matrix()
+    ip, ix, dv = m.device_arrays()
+    m2 = qa.SpMat.new_unchecked(m.shape(), dlpack.from_dlpack(dlpack.to_dlpack(dv)), ix, ip)
+    assert m2.device_arrays()[2].data_ptr() == dv.data_ptr()
+    x = np.arange(4) + 1j
+    assert np.array_equal(qa.spmat_dot_densevec(m2, x), qa.spmat_dot_densevec(m, x))
+    with pytest.raises(qa.QrError):
+        qa.SpMat.new_unchecked(m.shape(), dv.to(torch.complex64), ix, ip)
