@@ -135,7 +135,7 @@ void build_patterns(const qr_op *op, int L, int tb, uint32_t pcap, uint32_t ucap
         if (g1 > g0) {
             auto cls = [far_mask](const ApplyGroup &a) {
                 return ((a.m & far_mask) ? 1u : 0u) | ((a.u >= 0 || a.ps != a.pk) ? 2u : 0u) |
-                       (a.real ? 0u : 4u);
+                       ((a.flags & kGroupReal) ? 0u : 4u);
             };
             std::stable_sort(out.grp.begin() + g0, out.grp.end(),
                              [&cls](const ApplyGroup &a, const ApplyGroup &b) { return cls(a) < cls(b); });
@@ -201,7 +201,7 @@ void build_patterns(const qr_op *op, int L, int tb, uint32_t pcap, uint32_t ucap
             }
             a.u = has_u ? int32_t(uint32_t(out.ug.size()) - cur.u0) : -1;
             if (has_u) out.ug.push_back(uint32_t(out.grp.size()));
-            a.real = real ? 1u : 0u;
+            a.flags = real ? kGroupReal : 0u;
             out.grp.push_back(a);
         }
     }

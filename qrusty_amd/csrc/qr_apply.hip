@@ -198,6 +198,59 @@ __device__ __forceinline__ void accumulate(const ApplyGroup &gd, uint32_t tid, u
         return;
     }
     const double2 *su = sU + (gd.u >= 0 ? uint32_t(gd.u) : zero_row) * R;
+    const uint32_t nst = gd.pk - gd.ps;
+    if (nst <= 2) {
+        // at most two straddling patterns: each one's per-thread coefficient
+        // (the sign of the thread bits applied) is formed once, and per row
+        // only the wave-uniform sign of the row-index bits flips it
+        auto rows = [&](auto nst_t) {
+            constexpr uint32_t kN = decltype(nst_t)::value;
+            double2 aq[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+            uint32_t zh[2] = {0u, 0u};
+#pragma unroll
+            for (uint32_t j = 0; j < kN; ++j) {
+                const double2 a = sA[gd.ps + j];
+                const uint32_t zl = uint32_t(__builtin_amdgcn_readfirstlane(int(sZ[gd.ps + j])));
+                const uint32_t f = uint32_t(__popc(tid & zl)) << 31;
+                aq[j] = make_double2(flip_sign(a.x, f), flip_sign(a.y, f));
+                zh[j] = zl >> kApplyThreadBits;
+            }
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                if (!kFull && tid + uint64_t(k) * kApplyThreads >= tile_rows) break;
+                const double2 uk = su[k];
+                double dr = cr + uk.x, di = cplx ? ci + uk.y : 0.0;
+#pragma unroll
+                for (uint32_t j = 0; j < kN; ++j) {
+                    const uint32_t s = uint32_t(__popc(uint32_t(k) & zh[j])) << 31;
+                    dr += flip_sign(aq[j].x, s);
+                    if (cplx) di += flip_sign(aq[j].y, s);
+                }
+                const cT xb = xg[k];
+                if (!kMixed) {
+                    ar[k] = fma(dr, xb.x, ar[k]);
+                    ai[k] = fma(dr, xb.y, ai[k]);
+                    if (cplx) {
+                        ar[k] = fma(-di, xb.y, ar[k]);
+                        ai[k] = fma(di, xb.x, ai[k]);
+                    }
+                } else {
+                    const float fr = float(dr);
+                    br[k] = fmaf(fr, xb.x, br[k]);
+                    bi[k] = fmaf(fr, xb.y, bi[k]);
+                    if (cplx) {
+                        const float fi = float(di);
+                        br[k] = fmaf(-fi, xb.y, br[k]);
+                        bi[k] = fmaf(fi, xb.x, bi[k]);
+                    }
+                }
+            }
+        };
+        if (nst == 0) rows(std::integral_constant<uint32_t, 0>{});
+        else if (nst == 1) rows(std::integral_constant<uint32_t, 1>{});
+        else rows(std::integral_constant<uint32_t, 2>{});
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         if (!kFull && tid + uint64_t(k) * kApplyThreads >= tile_rows) break;
@@ -378,7 +431,8 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_tiled(
                     const ApplyGroup gd = grp[g];
                     // all descriptor fields in SGPRs before the gathers are issued,
                     // so that no later scalar-load wait also waits for them
-                    asm volatile("" ::"s"(gd.p0), "s"(gd.ps), "s"(gd.pk), "s"(gd.p1), "s"(gd.u));
+                    asm volatile("" ::"s"(gd.p0), "s"(gd.ps), "s"(gd.pk), "s"(gd.p1), "s"(gd.u),
+                                 "s"(gd.flags));
                     gather<kFar, kFull, R>(gd.m, tid, tile_rows, base, xt, xv, xg);
                     accumulate<kMixed, kFull, kRowvar, kCplx, R>(gd, tid, tile_rows, zero_row, sA, sZ,
                                                                  sU, xg, ar, ai, br, bi);

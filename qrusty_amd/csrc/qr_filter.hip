@@ -10,8 +10,8 @@
 
 #include <hipcub/device/device_reduce.hpp>
 #include <hipcub/device/device_scan.hpp>
-#include <hipcub/iterator/counting_input_iterator.hpp>
-#include <hipcub/iterator/transform_input_iterator.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
+#include <rocprim/iterator/transform_iterator.hpp>
 
 namespace qr {
 
@@ -28,7 +28,7 @@ struct KeepFlag {
     }
 };
 
-using FlagIter = hipcub::TransformInputIterator<uint64_t, KeepFlag, hipcub::CountingInputIterator<uint64_t>>;
+using FlagIter = rocprim::transform_iterator<rocprim::counting_iterator<uint64_t>, KeepFlag, uint64_t>;
 
 __global__ __launch_bounds__(256) void k_filter_scatter(uint64_t nnz, const double2 *__restrict__ d,
                                                         const uint64_t *__restrict__ ix,
@@ -59,7 +59,7 @@ static unsigned blocks_for(uint64_t n) { return unsigned((n + 255) / 256); }
 int launch_count_zeros(uint64_t nnz, const double *d, double tol, uint64_t *count, hipStream_t s) {
     *count = 0;
     if (nnz == 0) return QR_OK;
-    FlagIter it(hipcub::CountingInputIterator<uint64_t>(0),
+    FlagIter it(rocprim::counting_iterator<uint64_t>(0),
                 KeepFlag{reinterpret_cast<const double2 *>(d), nnz, tol, true});
     size_t tmp_bytes = 0;
     QR_HIP_CHECK(hipcub::DeviceReduce::Sum(nullptr, tmp_bytes, it, (uint64_t *)nullptr, nnz, s));
@@ -80,7 +80,7 @@ int launch_eliminate_zeros(uint64_t nrows, uint64_t nnz, const uint64_t *ip, con
                            uint64_t *nnz_out, hipStream_t s) {
     // pos[0..nnz]: exclusive scan of the keep flags over nnz + 1 items (the
     // last flag is 0, so pos[nnz] is the kept total)
-    FlagIter it(hipcub::CountingInputIterator<uint64_t>(0),
+    FlagIter it(rocprim::counting_iterator<uint64_t>(0),
                 KeepFlag{reinterpret_cast<const double2 *>(d), nnz, tol, false});
     size_t tmp_bytes = 0;
     QR_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, it, (uint64_t *)nullptr,

@@ -54,8 +54,9 @@ struct ApplyGroup {
     uint64_t m;
     uint32_t p0, ps, pk, p1;
     int32_t u;
-    uint32_t real;  // every coefficient of the group is real
+    uint32_t flags;  // kGroupReal
 };
+constexpr uint32_t kGroupReal = 1;  // every coefficient of the group is real
 
 // Groups whose patterns [p0, p1) and table rows [u0, u1) are staged in LDS
 // together.  Inside a chunk the groups are ordered by class -- bit 0: the mask

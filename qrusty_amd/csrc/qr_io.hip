@@ -337,7 +337,7 @@ int raw_header(Source &src, bool &swab, uint64_t h[4], uint64_t &lp) {
     if (m[0] == 'M' && m[1] == 'I') swab = false;
     else if (m[0] == 'I' && m[1] == 'M') swab = true;
     else return fail(QR_ERR_FORMAT, "raw read: bad byte-order mark");
-    uint64_t v[5];
+    uint64_t v[4];  // storage, rows, cols, indptr length
     if (src.read(v, sizeof v) != sizeof v) return fail(QR_ERR_FORMAT, "raw read: file too short");
     if (swab)
         for (auto &x : v) x = __builtin_bswap64(x);
