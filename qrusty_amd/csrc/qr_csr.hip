@@ -228,6 +228,7 @@ __device__ __forceinline__ T mv_load(const T *p) {
     return __builtin_nontemporal_load(p);
 }
 
+
 template <int LPR>
 __global__ __launch_bounds__(256) void k_csr_matvec(uint64_t nrows, uint64_t ncols, uint64_t nnz,
                                                     uint32_t nblocks_xcd,
@@ -266,15 +267,31 @@ __global__ __launch_bounds__(256) void k_csr_matvec(uint64_t nrows, uint64_t nco
                 acc.y = fma(a[u].y, bx.x, acc.y);
             }
         }
-        for (; k < k1; k += LPR) {
-            const uint64_t c = mv_load(indices + k);
-            const dvec2 av = mv_load(reinterpret_cast<const dvec2 *>(data + k));
-            double2 bx = make_double2(0.0, 0.0);
-            if (c < ncols) bx = x[c];
-            acc.x = fma(av.x, bx.x, acc.x);
-            acc.x = fma(-av.y, bx.y, acc.x);
-            acc.y = fma(av.x, bx.y, acc.y);
-            acc.y = fma(av.y, bx.x, acc.y);
+        if (k < k1) {
+            // the fewer than U entries left (all of a short row): predicated,
+            // with every load issued before the first is used
+            uint64_t c[U];
+            double2 a[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                c[u] = ~uint64_t(0);
+                a[u] = make_double2(0.0, 0.0);
+                if (k + u * LPR < k1) {
+                    c[u] = mv_load(indices + k + u * LPR);
+                    const dvec2 av = mv_load(reinterpret_cast<const dvec2 *>(data + k + u * LPR));
+                    a[u] = make_double2(av.x, av.y);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (c[u] < ncols) {
+                    const double2 bx = x[c[u]];
+                    acc.x = fma(a[u].x, bx.x, acc.x);
+                    acc.x = fma(-a[u].y, bx.y, acc.x);
+                    acc.y = fma(a[u].x, bx.y, acc.y);
+                    acc.y = fma(a[u].y, bx.x, acc.y);
+                }
+            }
         }
     }
 #pragma unroll
@@ -290,7 +307,8 @@ int launch_csr_matvec(uint64_t nrows, uint64_t ncols, uint64_t nnz, const uint64
                       double *d_y, hipStream_t s) {
     const uint64_t avg = nrows ? nnz / nrows : 0;
     // ~4+ entries per lane, at most 8 lanes per row (8 rows per wave; 4 and 16
-    // to 32 lanes per row measured slower at G = 2000)
+    // to 32 lanes per row measured slower at G = 2000, and 16 no faster at
+    // G = 25)
     int lpr = 1;
     while (lpr < 8 && uint64_t(lpr) * 4 <= avg) lpr *= 2;
     const uint64_t threads = nrows * uint64_t(lpr);
