@@ -411,9 +411,14 @@ def bench_vec(args, torch, qa):
     t_pre = timed(lambda: L.check(L.lib.qr_precond2(n, P(x), P(y), e, 1e-14, P(z), sh)), reps)
     del x, y, z
     labels, coeffs = workloads.heisenberg_ring(24, seed=workloads.SEED_CFG3)
-    m = qa.SparsePauliOp([qa.Pauli(l) for l in labels], coeffs).to_matrix_mode("Hip")
+    sop = qa.SparsePauliOp([qa.Pauli(l) for l in labels], coeffs)
+    m = sop.to_matrix_mode("Hip")
     ip, ix, dv = m.device_arrays()
     N = 1 << 24
+    # the build of that CSR (24 B per entry written; its diagonal group has 48
+    # terms, the other 24 groups 2)
+    h = sop._handle(0)
+    t_build = timed(lambda: L.check(L.lib.qr_csr_build(h.h, P(ip), P(ix), P(dv), sh)), reps)
     d = torch.empty(N, dtype=torch.complex128, device="cuda")
     t_diag = timed(lambda: L.check(L.lib.qr_csr_diagonal(N, N, dv.numel(), P(ip), P(ix), P(dv),
                                                          P(d), sh)), reps)
@@ -448,6 +453,8 @@ def bench_vec(args, torch, qa):
            "axpby_ms": t_axpby, "axpby_GBps": 48 * n / t_axpby / 1e6,
            "precond2_ms": t_pre, "precond2_GBps": 48 * n / t_pre / 1e6,
            "diagonal_ms": t_diag, "diagonal_rows_per_s": N / (t_diag / 1e3),
+           "heisenberg24_csr_build_ms": t_build,
+           "heisenberg24_csr_build_GBps": (24 * dv.numel() + 8 * (N + 1)) / t_build / 1e6,
            "heisenberg24_csr_matvec_ms": t_mv, "heisenberg24_csr_matvec_GBps": mv_bytes / t_mv / 1e6,
            "heisenberg24_csr_matvec_frac": mv_bytes / t_mv / 1e6 / HBM_PEAK_GBPS,
            "spadd_count_ms": t_cnt, "spadd_fill_ms": t_fill, "spadd_nnz_in": nin,
@@ -455,6 +462,7 @@ def bench_vec(args, torch, qa):
     log(f"vec: axpby {t_axpby:.3f} ms ({res['axpby_GBps']:.0f} GB/s), precond2 {t_pre:.3f} ms "
         f"({res['precond2_GBps']:.0f} GB/s), diagonal {t_diag:.3f} ms ({res['diagonal_rows_per_s']:.3e} rows/s), "
         f"H-eI count {t_cnt:.3f} + fill {t_fill:.3f} ms ({res['spadd_GBps']:.0f} GB/s algorithmic), "
+        f"Heisenberg-24 CSR build {t_build:.3f} ms ({res['heisenberg24_csr_build_GBps']:.0f} GB/s), "
         f"Heisenberg-24 CSR matvec {t_mv:.3f} ms ({res['heisenberg24_csr_matvec_GBps']:.0f} GB/s, "
         f"{100 * res['heisenberg24_csr_matvec_frac']:.1f}% of HBM peak)")
     del m, ip, ix, dv, d, eip, eix, edv, ipo, ixo, dvo

@@ -99,7 +99,7 @@ static void build_rank_lists(const std::vector<uint64_t> &gx, std::vector<uint32
 }
 
 static void free_tables(DeviceTables &t) {
-    void *ptrs[] = {t.gx, t.gterm, t.tz, t.tc, t.rl_ptr, t.rl, t.gkind, t.partials};
+    void *ptrs[] = {t.gx, t.gterm, t.tz, t.tc, t.rl_ptr, t.rl, t.gkind, t.heavy, t.partials};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &pt : t.pat) {
@@ -230,10 +230,12 @@ int device_tables(const qr_op *op, DeviceTables **out) {
         if ((rc = upload(&t.gx, op->gx)) || (rc = upload(&t.gterm, op->gterm)) ||
             (rc = upload(&t.tz, op->tz)) || (rc = upload(&t.tc, tc2)) ||
             (rc = upload(&t.rl_ptr, op->rl_ptr)) || (rc = upload(&t.rl, op->rl)) ||
-            (rc = upload(&t.gkind, op->gkind))) {
+            (rc = upload(&t.gkind, op->gkind)) ||
+            (!op->heavy.empty() && (rc = upload(&t.heavy, op->heavy)))) {
             free_tables(t);
             return rc;
         }
+        t.nheavy = uint32_t(op->heavy.size());
         t.ready = true;
     }
     *out = &t;
@@ -356,6 +358,21 @@ int qr_op_create(const char *const *labels, const double *coeffs, uint64_t num_t
         op->gkind[g] = real ? 1 : (imag ? 2 : 0);
     }
     build_rank_lists(op->gx, op->rl_ptr, op->rl);
+    // heavy groups: at least 8 terms and 4x the mean, the largest kMaxHeavy
+    {
+        const double mean = double(op->T) / double(op->G);
+        std::vector<std::pair<uint32_t, uint32_t>> cand;  // (terms, group)
+        for (uint64_t g = 0; g < op->G; ++g) {
+            const uint32_t n = op->gterm[g + 1] - op->gterm[g];
+            if (n >= 8 && double(n) >= 4.0 * mean) cand.emplace_back(n, uint32_t(g));
+        }
+        std::sort(cand.begin(), cand.end(), [](const auto &a, const auto &b) {
+            return a.first != b.first ? a.first > b.first : a.second < b.second;
+        });
+        if (cand.size() > kMaxHeavy) cand.resize(kMaxHeavy);
+        for (const auto &c : cand) op->heavy.push_back(c.second);
+        std::sort(op->heavy.begin(), op->heavy.end());
+    }
     *out = op;
     return QR_OK;
 }

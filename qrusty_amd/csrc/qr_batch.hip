@@ -240,7 +240,7 @@ int make_plan(qr_batch *b, uint64_t *const *d_indptr, uint64_t *const *d_indices
                       has_csr ? reinterpret_cast<double2 *>(d_data[k]) : nullptr,
                       d_indptr ? d_indptr[k] : nullptr,
                       reinterpret_cast<const double2 *>(d_x[k]), reinterpret_cast<double2 *>(d_y[k]),
-                      0, N, G, jw, (G + (1u << jw) - 1) >> jw, dbuf};
+                      0, N, G, jw, (G + (1u << jw) - 1) >> jw, dbuf, nullptr, 0};
         d.tiles = ((N - 1) >> tl2) + 1;
         d.narrow = 1;
         d.nch = d.op.nchunks;

@@ -24,6 +24,8 @@
 #include "qr_internal.h"
 #include "qr_tile.h"
 
+#include <type_traits>
+
 namespace qr {
 
 constexpr int kBuildThreads = 256;
@@ -141,10 +143,10 @@ __global__ __launch_bounds__(kBuildThreads) void k_csr_build(
 
 // LDS-staged build (used when G <= kStageMaxGLean): one block per tile of the
 // shared tile builder (qr_tile.h), without the fused matvec.
-template <bool kNarrow, int kChunks>
+template <bool kNarrow, int kChunks, bool kHeavy>
 __global__ __launch_bounds__(kStageThreads) void k_csr_build_staged(TileOp op) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    build_tile_staged<kNarrow, kChunks, false>(op, blockIdx.x, smem_raw);
+    build_tile_staged<kNarrow, kChunks, false, kHeavy>(op, blockIdx.x, smem_raw);
 }
 
 __global__ __launch_bounds__(256) void k_indptr(uint64_t row_begin, uint64_t nrows, uint64_t G,
@@ -173,18 +175,22 @@ int launch_csr_build(const qr_op *op, const DeviceTables *t, uint64_t row_begin,
         const uint64_t nt = ((row_end - 1) >> tl2) - (row_begin >> tl2) + 1;
         if (nt > 0x7FFFFFFFull) return fail(QR_ERR_SIZE, "CSR build grid too large");
         uint32_t dbuf = 0;
-        const size_t shmem = stage_bytes(G, jw, &dbuf);
+        const uint32_t nh = heavy_fit(t->nheavy, jw);
+        const size_t shmem = stage_bytes(G, jw, &dbuf) + heavy_bytes(nh, jw);
         TileOp top{t->gx, t->gterm, t->tz, t->tc, t->rl_ptr, t->rl, d_indices,
                    reinterpret_cast<double2 *>(d_data), nullptr, nullptr, nullptr,
-                   row_begin, row_end, G, jw, nch, dbuf};
-        auto kst = narrow ? (nch <= 1 ? k_csr_build_staged<true, 1>
-                             : nch <= 2 ? k_csr_build_staged<true, 2>
-                             : nch <= uint32_t(kStageMaxChunks) ? k_csr_build_staged<true, kStageMaxChunks>
-                                                                : k_csr_build_staged<true, kStageLeanChunks>)
-                          : (nch <= 1 ? k_csr_build_staged<false, 1>
-                             : nch <= 2 ? k_csr_build_staged<false, 2>
-                             : nch <= uint32_t(kStageMaxChunks) ? k_csr_build_staged<false, kStageMaxChunks>
-                                                                : k_csr_build_staged<false, kStageLeanChunks>);
+                   row_begin, row_end, G, jw, nch, dbuf, t->heavy, nh};
+        auto pick = [&](auto narrow_t, auto heavy_t) {
+            constexpr bool kN = decltype(narrow_t)::value, kH = decltype(heavy_t)::value;
+            return nch <= 1 ? k_csr_build_staged<kN, 1, kH>
+                   : nch <= 2 ? k_csr_build_staged<kN, 2, kH>
+                   : nch <= uint32_t(kStageMaxChunks) ? k_csr_build_staged<kN, kStageMaxChunks, kH>
+                                                      : k_csr_build_staged<kN, kStageLeanChunks, kH>;
+        };
+        using T = std::true_type;
+        using F = std::false_type;
+        auto kst = narrow ? (nh ? pick(T{}, T{}) : pick(T{}, F{}))
+                          : (nh ? pick(F{}, T{}) : pick(F{}, F{}));
         QR_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(kst),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem)));
         hipLaunchKernelGGL(kst, dim3(unsigned(nt)), dim3(kStageThreads), shmem, s, top);

@@ -37,6 +37,7 @@ int hip_fail(hipError_t e, const char *what);
     } while (0)
 
 constexpr int kMaxDevices = 16;
+constexpr uint32_t kMaxHeavy = 4;  // heavy groups the CSR build folds per row
 
 // Row-ordering entry: while building row r, a group whose sorted-x position is j
 // moves by `sigma` positions when bit `bit` of r is set (see build_rank_lists).
@@ -80,6 +81,8 @@ struct DeviceTables {
     uint32_t *rl_ptr = nullptr;  // [G+1]
     RankEntry *rl = nullptr;     // [E]
     uint8_t *gkind = nullptr;    // [G]   0 complex, 1 real, 2 imaginary coefficients
+    uint32_t *heavy = nullptr;   // [nheavy] heavy groups (qr_op::heavy)
+    uint32_t nheavy = 0;
     double *partials = nullptr;  // workspace for <x|y> block partials
     size_t partials_cap = 0;     // in doubles
     // matrix-free apply tables, one set per precision (tile size)
@@ -129,6 +132,9 @@ struct qr_op {
     std::vector<uint8_t> gkind;
     std::vector<uint32_t> rl_ptr;
     std::vector<qr::RankEntry> rl;
+    // groups with many more terms than the others (at most kMaxHeavy,
+    // ascending): the CSR build folds them once per tile row, not per lane
+    std::vector<uint32_t> heavy;
     // device state
     mutable std::mutex mu;
     mutable qr::DeviceTables dev[qr::kMaxDevices];

@@ -65,6 +65,8 @@ struct TileOp {
     double2 *y;
     uint64_t row_begin, row_end;
     uint32_t G, jw_log2, nchunks, dbuf;
+    const uint32_t *heavy;  // groups folded once per row into LDS (ascending)
+    uint32_t nheavy;        // 0: none
 };
 
 // LDS bytes for one operator's stage: (one or two) x (index + value) x batch,
@@ -84,13 +86,41 @@ __host__ __device__ inline size_t stage_bytes(uint32_t G, uint32_t jw_log2, uint
     return (dbuf ? 2 * one : one) + 2 * 16 * sizeof(double2);
 }
 
+// LDS bytes of the heavy-group values: one double2 per row of the tile and
+// heavy group, placed after the stage.  At most kHeavyMaxBytes; the launch
+// takes as many heavy groups as fit.
+constexpr size_t kHeavyMaxBytes = 32 * 1024;
+__host__ __device__ inline size_t heavy_bytes(uint32_t nheavy, uint32_t jw_log2) {
+    return size_t(nheavy) * (size_t(kBuildM) << (10 - jw_log2)) * sizeof(double2);
+}
+__host__ __device__ inline uint32_t heavy_fit(uint32_t nheavy, uint32_t jw_log2) {
+    while (nheavy && heavy_bytes(nheavy, jw_log2) > kHeavyMaxBytes) --nheavy;
+    return nheavy;
+}
+
+// Term-order fold of group j in row r: (-1)^popcount(r & z_t) c'_t, the
+// first value, then + (accel.rs:191-208).
+template <bool kNarrow>
+__device__ __forceinline__ double2 fold_group(const TileOp &op, uint32_t j, uint64_t r) {
+    const uint32_t t0 = op.gterm[j], t1 = op.gterm[j + 1];
+    double2 v = signed_coeff(op.tc[t0], parity_and<kNarrow>(r, op.tz[t0]));
+    for (uint32_t t = t0 + 1; t < t1; ++t) {
+        const double2 ct = signed_coeff(op.tc[t], parity_and<kNarrow>(r, op.tz[t]));
+        v.x = v.x + ct.x;
+        v.y = v.y + ct.y;
+    }
+    return v;
+}
+
 __host__ __device__ inline uint32_t stage_jw_log2(uint32_t G) {
     uint32_t jw = 0;
     while ((1u << jw) < G && jw < 10) ++jw;
     return jw;
 }
 
-template <bool kNarrow, int kChunks, bool kMatvec>
+// kHeavy: the operator has heavy groups (op.nheavy > 0); a separate
+// instantiation, so that operators without them run the plain loop.
+template <bool kNarrow, int kChunks, bool kMatvec, bool kHeavy = false>
 __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t tile,
                                                   unsigned char *smem_raw) {
     // the second term of each group stays in registers when few chunks leave
@@ -117,6 +147,8 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
     double2 *red = reinterpret_cast<double2 *>(op.dbuf ? reinterpret_cast<unsigned char *>(st_idx[1]) +
                                                              size_t(batch_pad) * 24
                                                        : reinterpret_cast<unsigned char *>(st_idx[1]));
+    // heavy-group values of the tile's rows: hv[h << tile_log2 | (r - tile_base)]
+    double2 *hv = reinterpret_cast<double2 *>(smem_raw + stage_bytes(G, jw_log2, nullptr));
 
     // rows of the tile: indptr
     if (op.indptr) {
@@ -136,10 +168,15 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
     mask_t xj[kChunks], z0[kChunks], z1[kChunks];
     double2 c0[kChunks], c1[kChunks];
     uint32_t t0[kChunks], t1[kChunks];
+    int32_t hs[kChunks];  // heavy slot of group j, or -1
 #pragma unroll
     for (int c = 0; c < kChunks; ++c) {
         const uint32_t j = (uint32_t(c) << jw_log2) + lane;
         rfx[c] = int32_t(j);
+        hs[c] = -1;
+        if (kHeavy)
+            for (uint32_t h = 0; h < op.nheavy; ++h)
+                if (op.heavy[h] == j) hs[c] = int32_t(h);
         int32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
         s01[c] = s23[c] = 0;
         xj[c] = z0[c] = z1[c] = 0;
@@ -172,6 +209,18 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
         }
     }
 
+    if (kHeavy) {
+        // heavy groups: one thread per (group, row) folds the whole group, so
+        // no lane of a batch waits on a long term chain
+        const uint32_t tr = uint32_t(kBuildM) << rs_log2;
+        for (uint32_t i = tid; i < op.nheavy * tr; i += kStageThreads) {
+            const uint64_t r = tile_base + (i & (tr - 1));
+            if (r >= row_begin && r < row_end)
+                hv[i] = fold_group<kNarrow>(op, op.heavy[i / tr], r);
+        }
+        __syncthreads();
+    }
+
 #pragma unroll 1
     for (int m = 0; m < kBuildM; ++m) {
         const int buf = op.dbuf ? (m & 1) : 0;
@@ -194,7 +243,9 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
             // term-order fold: first value, then + (accel.rs:191-208)
             double2 v = kLean ? signed_coeff(op.tc[tf], parity_and<kNarrow>(r, op.tz[tf]))
                               : signed_coeff(c0[c], parity_and<kNarrow>(r, z0[c]));
-            if (t1[c] > tf + 1) {
+            if (kHeavy && hs[c] >= 0) {
+                v = hv[(uint32_t(hs[c]) << tile_log2) | (uint32_t(m) << rs_log2) | ro];
+            } else if (t1[c] > tf + 1) {
                 const double2 e = kHold2 ? signed_coeff(c1[c], parity_and<kNarrow>(r, z1[c]))
                                          : signed_coeff(op.tc[tf + 1],
                                                         parity_and<kNarrow>(r, op.tz[tf + 1]));
