@@ -358,6 +358,26 @@ int qr_op_create(const char *const *labels, const double *coeffs, uint64_t num_t
         op->gkind[g] = real ? 1 : (imag ? 2 : 0);
     }
     build_rank_lists(op->gx, op->rl_ptr, op->rl);
+    // subtrees of at most kSubMaxG groups, split where build_rank_lists splits
+    if (op->G > kSubMaxG) {
+        struct Node { uint32_t lo, hi, depth; };
+        std::vector<Node> stack{{0, uint32_t(op->G), 0}};
+        while (!stack.empty()) {
+            const Node nd = stack.back();
+            stack.pop_back();
+            if (nd.hi - nd.lo <= kSubMaxG) {
+                op->leaves.push_back({nd.lo, nd.hi, nd.depth});
+                continue;
+            }
+            const uint64_t bit = uint64_t(1) << (63 - __builtin_clzll(op->gx[nd.lo] ^ op->gx[nd.hi - 1]));
+            const uint32_t m = uint32_t(std::partition_point(op->gx.begin() + nd.lo,
+                                                             op->gx.begin() + nd.hi,
+                                                             [bit](uint64_t v) { return (v & bit) == 0; }) -
+                                        op->gx.begin());
+            stack.push_back({m, nd.hi, nd.depth + 1});
+            stack.push_back({nd.lo, m, nd.depth + 1});
+        }
+    }
     // heavy groups: at least 8 terms and 4x the mean, the largest kMaxHeavy
     {
         const double mean = double(op->T) / double(op->G);

@@ -10,7 +10,9 @@
 // y = A x from the entries still in registers.  No item depends on another,
 // so every block leaves the loop as soon as the counter passes the end.
 // Operators whose G exceeds the largest LDS stage (kStageMaxGLean) are built
-// and applied by the single-operator kernels, recorded in the same graph.
+// and applied by the single-operator kernels, recorded in the same graph; in
+// the split form so are those with G > kSubMaxG (built as sub-operators of at
+// most kSubMaxG groups, two blocks per CU, then applied by k_batch_mv).
 //
 // Tuning knobs (qr_tuning): tiles per item (work granularity), resident
 // blocks per CU (grid size), and the item order (input order or largest
@@ -222,17 +224,26 @@ int make_plan(qr_batch *b, uint64_t *const *d_indptr, uint64_t *const *d_indices
         int rc = device_tables(op, &t);
         if (rc) return rc;
         const uint32_t G = uint32_t(op->G);
-        if (op->G > kStageMaxGLean) {
+        const uint64_t N = uint64_t(1) << op->nq;
+        if (op->G > kStageMaxGLean || (plan.split && !op->leaves.empty())) {
+            // built by the single-operator kernels (sub-operators of at most
+            // kSubMaxG groups); in the split form applied by k_batch_mv
             if (!has_csr || !d_indptr || !d_indptr[k])
                 return fail(QR_ERR_ARG, "batch operators with more than 6144 distinct x masks "
                                         "need CSR output buffers");
             plan.big.push_back(k);
+            if (plan.split) {
+                BatchMvDesc m{d_indices[k], reinterpret_cast<const double2 *>(d_data[k]),
+                              reinterpret_cast<const double2 *>(d_x[k]),
+                              reinterpret_cast<double2 *>(d_y[k]), N, G, plan.mv_items};
+                plan.mv_items += (N + kMvRows - 1) / kMvRows;
+                plan.mv.push_back(m);
+            }
             continue;
         }
         const uint32_t jw = stage_jw_log2(G);
         uint32_t dbuf = 0;
         const size_t sb = stage_bytes(G, jw, &dbuf);
-        const uint64_t N = uint64_t(1) << op->nq;
         const uint32_t tl2 = 10 - jw + 4;
         BatchOpDesc d{};
         d.op = TileOp{t->gx, t->gterm, t->tz, t->tc, t->rl_ptr, t->rl,
@@ -240,7 +251,7 @@ int make_plan(qr_batch *b, uint64_t *const *d_indptr, uint64_t *const *d_indices
                       has_csr ? reinterpret_cast<double2 *>(d_data[k]) : nullptr,
                       d_indptr ? d_indptr[k] : nullptr,
                       reinterpret_cast<const double2 *>(d_x[k]), reinterpret_cast<double2 *>(d_y[k]),
-                      0, N, G, jw, (G + (1u << jw) - 1) >> jw, dbuf, nullptr, 0};
+                      0, N, G, jw, (G + (1u << jw) - 1) >> jw, dbuf, nullptr, 0, 0, 0, G};
         d.tiles = ((N - 1) >> tl2) + 1;
         d.narrow = 1;
         d.nch = d.op.nchunks;
@@ -295,14 +306,8 @@ int record(qr_batch *b, const Plan &plan, const qr_tuning &tu, uint64_t *const *
     launch_class<true, kStageMaxChunks>(b, plan, 2, cap, tpi, s);
     launch_class<true, kStageLeanChunks>(b, plan, 3, cap, tpi, s);
     QR_HIP_CHECK(hipGetLastError());
-    if (plan.split && plan.mv_items) {
-        // 8 resident 256-thread blocks per CU and slot
-        const uint64_t grid = std::min<uint64_t>(uint64_t(b->ncu) * 8 * bpc, plan.mv_items);
-        hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, s, b->d_counter + kClasses);
-        hipLaunchKernelGGL(k_batch_mv, dim3(unsigned(grid)), dim3(256), 0, s, b->d_mv,
-                           uint32_t(plan.mv.size()), plan.mv_items, b->d_counter + kClasses);
-        QR_HIP_CHECK(hipGetLastError());
-    }
+    // operators built by the single-operator kernels (before the matvec pass
+    // that reads their CSR in the split form)
     for (uint32_t k : plan.big) {
         const qr_op *op = b->ops[k];
         DeviceTables *t;
@@ -311,9 +316,19 @@ int record(qr_batch *b, const Plan &plan, const qr_tuning &tu, uint64_t *const *
         const uint64_t N = uint64_t(1) << op->nq;
         rc = launch_csr_build(op, t, 0, N, d_indptr[k], d_indices[k], d_data[k], s);
         if (rc) return rc;
-        rc = launch_csr_matvec(N, N, N * op->G, d_indptr[k], d_indices[k], d_data[k], d_x[k],
-                               d_y[k], s);
-        if (rc) return rc;
+        if (!plan.split) {
+            rc = launch_csr_matvec(N, N, N * op->G, d_indptr[k], d_indices[k], d_data[k], d_x[k],
+                                   d_y[k], s);
+            if (rc) return rc;
+        }
+    }
+    if (plan.split && plan.mv_items) {
+        // 8 resident 256-thread blocks per CU and slot
+        const uint64_t grid = std::min<uint64_t>(uint64_t(b->ncu) * 8 * bpc, plan.mv_items);
+        hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, s, b->d_counter + kClasses);
+        hipLaunchKernelGGL(k_batch_mv, dim3(unsigned(grid)), dim3(256), 0, s, b->d_mv,
+                           uint32_t(plan.mv.size()), plan.mv_items, b->d_counter + kClasses);
+        QR_HIP_CHECK(hipGetLastError());
     }
     return QR_OK;
 }

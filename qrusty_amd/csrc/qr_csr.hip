@@ -143,10 +143,19 @@ __global__ __launch_bounds__(kBuildThreads) void k_csr_build(
 
 // LDS-staged build (used when G <= kStageMaxGLean): one block per tile of the
 // shared tile builder (qr_tile.h), without the fused matvec.
-template <bool kNarrow, int kChunks, bool kHeavy>
+template <bool kNarrow, int kChunks, int kMode>
 __global__ __launch_bounds__(kStageThreads) void k_csr_build_staged(TileOp op) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    build_tile_staged<kNarrow, kChunks, false, kHeavy>(op, blockIdx.x, smem_raw);
+    build_tile_staged<kNarrow, kChunks, false, kMode>(op, blockIdx.x, smem_raw);
+}
+
+// Sub-operators of at most 32 qubits (at most 2048 groups, a 48 KB stage):
+// held to 64 VGPRs, i.e. 8 waves per SIMD, so that two 1024-thread blocks
+// share a CU
+template <bool kNarrow, int kChunks>
+__global__ __launch_bounds__(kStageThreads, 8) void k_csr_build_sub(TileOp op) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    build_tile_staged<kNarrow, kChunks, false, kTileSub>(op, blockIdx.x, smem_raw);
 }
 
 __global__ __launch_bounds__(256) void k_indptr(uint64_t row_begin, uint64_t nrows, uint64_t G,
@@ -168,6 +177,32 @@ int launch_csr_build(const qr_op *op, const DeviceTables *t, uint64_t row_begin,
     }
     if (nrows == 0) return QR_OK;
     const bool narrow = op->nq <= 32;
+    if (!op->leaves.empty()) {
+        // one sub-operator per subtree of the mask trie, each <= kSubMaxG
+        // groups with one row per batch (JW = 1024): the fast 1- and 2-chunk
+        // kernels, each writing its contiguous segment of every row
+        const uint32_t jw = 10, tl2 = 4;
+        const uint64_t nt = ((row_end - 1) >> tl2) - (row_begin >> tl2) + 1;
+        if (nt > 0x7FFFFFFFull) return fail(QR_ERR_SIZE, "CSR build grid too large");
+        for (size_t k = 0; k < op->leaves.size(); ++k) {
+            const auto &lf = op->leaves[k];
+            const uint32_t g = lf.s1 - lf.s0, nch = (g + 1023) >> 10;
+            uint32_t dbuf = 0;
+            const size_t shmem = stage_bytes(g, jw, &dbuf);
+            TileOp top{t->gx + lf.s0, t->gterm + lf.s0, t->tz, t->tc, t->rl_ptr + lf.s0, t->rl,
+                       d_indices, reinterpret_cast<double2 *>(d_data), nullptr, nullptr, nullptr,
+                       row_begin, row_end, g, jw, nch, dbuf, nullptr, 0, lf.s0, lf.depth, G};
+            // (wide masks do not fit 64 VGPRs without spills: one block per CU)
+            auto kst = narrow ? (nch <= 1 ? k_csr_build_sub<true, 1> : k_csr_build_sub<true, 2>)
+                              : (nch <= 1 ? k_csr_build_staged<false, 1, kTileSub>
+                                          : k_csr_build_staged<false, 2, kTileSub>);
+            QR_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(kst),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem)));
+            hipLaunchKernelGGL(kst, dim3(unsigned(nt)), dim3(kStageThreads), shmem, s, top);
+            QR_HIP_CHECK(hipGetLastError());
+        }
+        return QR_OK;
+    }
     if (G <= kStageMaxGLean) {
         const uint32_t jw = stage_jw_log2(G);
         const uint32_t nch = (G + (1u << jw) - 1) >> jw;
@@ -179,9 +214,10 @@ int launch_csr_build(const qr_op *op, const DeviceTables *t, uint64_t row_begin,
         const size_t shmem = stage_bytes(G, jw, &dbuf) + heavy_bytes(nh, jw);
         TileOp top{t->gx, t->gterm, t->tz, t->tc, t->rl_ptr, t->rl, d_indices,
                    reinterpret_cast<double2 *>(d_data), nullptr, nullptr, nullptr,
-                   row_begin, row_end, G, jw, nch, dbuf, t->heavy, nh};
+                   row_begin, row_end, G, jw, nch, dbuf, t->heavy, nh, 0, 0, G};
         auto pick = [&](auto narrow_t, auto heavy_t) {
-            constexpr bool kN = decltype(narrow_t)::value, kH = decltype(heavy_t)::value;
+            constexpr bool kN = decltype(narrow_t)::value;
+            constexpr int kH = decltype(heavy_t)::value ? kTileHeavy : kTilePlain;
             return nch <= 1 ? k_csr_build_staged<kN, 1, kH>
                    : nch <= 2 ? k_csr_build_staged<kN, 2, kH>
                    : nch <= uint32_t(kStageMaxChunks) ? k_csr_build_staged<kN, kStageMaxChunks, kH>

@@ -38,6 +38,7 @@ int hip_fail(hipError_t e, const char *what);
 
 constexpr int kMaxDevices = 16;
 constexpr uint32_t kMaxHeavy = 4;  // heavy groups the CSR build folds per row
+constexpr uint32_t kSubMaxG = 2048;  // groups of one CSR-build sub-operator
 
 // Row-ordering entry: while building row r, a group whose sorted-x position is j
 // moves by `sigma` positions when bit `bit` of r is set (see build_rank_lists).
@@ -135,6 +136,13 @@ struct qr_op {
     // groups with many more terms than the others (at most kMaxHeavy,
     // ascending): the CSR build folds them once per tile row, not per lane
     std::vector<uint32_t> heavy;
+    // G > kSubMaxG: the mask trie cut into subtrees of at most kSubMaxG
+    // groups [s0, s1) at depth `depth`; the CSR build runs one sub-operator
+    // per subtree (qr_tile.h TileOp::gbase)
+    struct SubLeaf {
+        uint32_t s0, s1, depth;
+    };
+    std::vector<SubLeaf> leaves;
     // device state
     mutable std::mutex mu;
     mutable qr::DeviceTables dev[qr::kMaxDevices];

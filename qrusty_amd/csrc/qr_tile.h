@@ -67,6 +67,14 @@ struct TileOp {
     uint32_t G, jw_log2, nchunks, dbuf;
     const uint32_t *heavy;  // groups folded once per row into LDS (ascending)
     uint32_t nheavy;        // 0: none
+    // Sub-operator of a split operator: the groups [gbase, gbase + G) of a
+    // subtree of the mask trie.  In row r they occupy the contiguous ranks
+    // [off(r), off(r) + G), off(r) = gbase + the sigma of the subtree's
+    // nanc ancestors whose bit is set in r, i.e. the first nanc entries of
+    // its first group's rank list.  gx, gterm and rl_ptr point at group
+    // gbase; rows are gstride entries apart in the outputs.  An operator
+    // built whole has gbase = nanc = 0 and gstride = G.
+    uint32_t gbase, nanc, gstride;
 };
 
 // LDS bytes for one operator's stage: (one or two) x (index + value) x batch,
@@ -118,11 +126,15 @@ __host__ __device__ inline uint32_t stage_jw_log2(uint32_t G) {
     return jw;
 }
 
-// kHeavy: the operator has heavy groups (op.nheavy > 0); a separate
-// instantiation, so that operators without them run the plain loop.
-template <bool kNarrow, int kChunks, bool kMatvec, bool kHeavy = false>
+// kMode: kTilePlain; kTileHeavy, the operator has heavy groups
+// (op.nheavy > 0); kTileSub, a sub-operator (op.nanc, op.gbase, op.gstride).
+// Separate instantiations, so that plain operators run the plain loop (its
+// register count decides whether two blocks share a CU).
+constexpr int kTilePlain = 0, kTileHeavy = 1, kTileSub = 2;
+template <bool kNarrow, int kChunks, bool kMatvec, int kMode = kTilePlain>
 __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t tile,
                                                   unsigned char *smem_raw) {
+    constexpr bool kHeavy = kMode == kTileHeavy, kSub = kMode == kTileSub;
     // the second term of each group stays in registers when few chunks leave
     // room; with more than kStageMaxChunks not even the first one does
     constexpr bool kHold2 = kChunks <= 2;
@@ -153,11 +165,35 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
     // rows of the tile: indptr
     if (op.indptr) {
         const uint64_t tr = uint64_t(1) << tile_log2;
+        const uint64_t stride = kSub ? op.gstride : G;
         for (uint64_t i = tid; i < tr; i += kStageThreads) {
             const uint64_t r = tile_base + i;
-            if (r >= row_begin && r < row_end) op.indptr[r - row_begin] = r * G;
-            if (r + 1 == row_end) op.indptr[r + 1 - row_begin] = (r + 1) * G;
+            if (r >= row_begin && r < row_end) op.indptr[r - row_begin] = r * stride;
+            if (r + 1 == row_end) op.indptr[r + 1 - row_begin] = (r + 1) * stride;
         }
+    }
+    // a sub-operator's first rank in the row: fixed bits summed, batch bits
+    // as four offsets (like the rank lists below).  The same for every
+    // thread of the block -- a sub-operator has one row per batch (JW = 1024),
+    // a whole operator no ancestors -- so it is held in scalar registers.
+    int32_t off_fix = 0, o0 = 0, o1 = 0, o2 = 0, o3 = 0;
+    if (kSub) off_fix = int32_t(op.gbase);
+    for (uint32_t e = 0; kSub && e < op.nanc; ++e) {
+        const RankEntry en = op.rl[op.rl_ptr[0] + e];
+        const uint32_t b = uint32_t(en.bit);
+        const uint32_t q = b - rs_log2;  // wraps for b < rs_log2
+        if (q == 0) o0 = en.sigma;
+        else if (q == 1) o1 = en.sigma;
+        else if (q == 2) o2 = en.sigma;
+        else if (q == 3) o3 = en.sigma;
+        else if ((rfix >> b) & 1) off_fix += en.sigma;
+    }
+    if (kSub) {
+        off_fix = __builtin_amdgcn_readfirstlane(off_fix);
+        o0 = __builtin_amdgcn_readfirstlane(o0);
+        o1 = __builtin_amdgcn_readfirstlane(o1);
+        o2 = __builtin_amdgcn_readfirstlane(o2);
+        o3 = __builtin_amdgcn_readfirstlane(o3);
     }
 
     // masks of <= 32-qubit operators are held in 32 bits
@@ -172,7 +208,7 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
 #pragma unroll
     for (int c = 0; c < kChunks; ++c) {
         const uint32_t j = (uint32_t(c) << jw_log2) + lane;
-        rfx[c] = int32_t(j);
+        rfx[c] = int32_t(kSub ? op.gbase + j : j);
         hs[c] = -1;
         if (kHeavy)
             for (uint32_t h = 0; h < op.nheavy; ++h)
@@ -228,6 +264,9 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
         double2 *sd = st_dat[buf];
         const uint64_t r = rfix + (uint64_t(m) << rs_log2);
         const bool row_ok = r >= row_begin && r < row_end;
+        const int32_t off = kSub ? off_fix + ((m & 1) ? o0 : 0) + ((m & 2) ? o1 : 0) +
+                                       ((m & 4) ? o2 : 0) + ((m & 8) ? o3 : 0)
+                                 : 0;
 #pragma unroll
         for (int c = 0; c < kChunks; ++c) {
             const uint32_t j = (uint32_t(c) << jw_log2) + lane;
@@ -258,7 +297,7 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
                 }
             }
             const uint64_t col = r ^ uint64_t(xj[c]);
-            const uint32_t slot = (ro * G) + uint32_t(rank);
+            const uint32_t slot = (ro * G) + uint32_t(rank - off);
             si[slot] = col;
             sd[slot] = v;
         }
@@ -295,9 +334,12 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
             }
         }
         if (op.indices && lo_r < hi_r) {
+            // consecutive rows are consecutive in the output unless this is a
+            // sub-operator, which always has one row per batch
             const uint32_t count = uint32_t(hi_r - lo_r) * G;
             const uint32_t src = uint32_t(lo_r - b0) * G;
-            const uint64_t dst = (lo_r - row_begin) * G;
+            const uint64_t dst = kSub ? (lo_r - row_begin) * op.gstride + uint32_t(off)
+                                      : (lo_r - row_begin) * G;
             for (uint32_t i = tid; i < count; i += kStageThreads) {
                 op.indices[dst + i] = si[src + i];
                 const double2 dv = sd[src + i];
