@@ -13,9 +13,11 @@
 //   value    = fold over the group's terms, in input order, of
 //              (-1)^popcount(r & z_t) * c'_t  (first value, then +)
 // which is exactly the reference's stable-sort-then-fold result, so indices
-// and values are bit-identical.  The kernel is bound by HBM writes
-// (24 B per nnz); consecutive lanes take consecutive groups of one row, so a
-// wave's stores land in a few contiguous runs of that row.
+// and values are bit-identical.  The kernel is bound by HBM writes (24 B per
+// nnz).  Rows are staged in LDS by the tile builder (qr_tile.h) and copied out
+// coalesced; an operator with more than kSubMaxG groups is built as
+// sub-operators, subtrees of the mask trie that fill a contiguous segment of
+// every row (qr_tile.h TileOp::gbase).
 //
 // CSR matvec (replaces accel::rowwise::spmat_dot_densevec, accel.rs:338-370):
 // a power-of-two team of lanes per row streams the row's indices/values with
@@ -28,121 +30,8 @@
 
 namespace qr {
 
-constexpr int kBuildThreads = 256;
-
-// Thread layout: a block has 256 threads = RS rows x JW groups (JW a power of
-// two >= min(G, 256)) and owns one tile of R = RS*16 aligned rows; it walks
-// the groups in chunks of JW.  Thread (ro, lane) owns group j = chunk*JW + lane
-// and rows r = tile_base + ro + m*RS, m = 0..15.  Every bit of r except the
-// four m bits is fixed for the thread, so its rank list is walked once per
-// tile: entries on fixed bits are summed into rank_fixed, the (at most four)
-// entries on m bits go into an offset table.  The group's first two terms are
-// held in registers; longer groups read the rest from the (cached) tables.
-// One block writes whole rows, so every cache line of a row is completed in
-// one XCD's L2 before it goes to HBM.
-#ifndef QR_BUILD_NT
-#define QR_BUILD_NT 0
-#endif
-#ifndef QR_BUILD_LOOP_CHUNKS
-#define QR_BUILD_LOOP_CHUNKS 1
-#endif
-
-template <typename T>
-__device__ __forceinline__ void store_out(T v, T *p) {
-#if QR_BUILD_NT
-    __builtin_nontemporal_store(v, p);
-#else
-    *p = v;
-#endif
-}
-
-template <bool kNarrow>
-__device__ __forceinline__ void build_group_tile(
-    uint32_t j, uint64_t rfix, uint32_t rs_log2, uint64_t row_begin, uint64_t row_end, uint32_t G,
-    const uint64_t *__restrict__ gx, const uint32_t *__restrict__ gterm,
-    const uint64_t *__restrict__ tz, const double2 *__restrict__ tc,
-    const uint32_t *__restrict__ rl_ptr, const RankEntry *__restrict__ rl,
-    uint64_t *__restrict__ indices, double2 *__restrict__ data) {
-    int32_t rank_fixed = int32_t(j);
-    int32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-    const uint32_t e1 = rl_ptr[j + 1];
-    for (uint32_t e = rl_ptr[j]; e < e1; ++e) {
-        const RankEntry en = rl[e];
-        const uint32_t b = uint32_t(en.bit);
-        const uint32_t q = b - rs_log2;  // wraps for b < rs_log2
-        if (q == 0) s0 = en.sigma;
-        else if (q == 1) s1 = en.sigma;
-        else if (q == 2) s2 = en.sigma;
-        else if (q == 3) s3 = en.sigma;
-        else if ((rfix >> b) & 1) rank_fixed += en.sigma;
-    }
-    const uint64_t xj = gx[j];
-    const uint32_t t0 = gterm[j], t1 = gterm[j + 1];
-    const uint64_t z0 = tz[t0];
-    const double2 c0 = tc[t0];
-    const bool multi = t1 > t0 + 1;
-    uint64_t z1 = 0;
-    double2 c1 = make_double2(0.0, 0.0);
-    if (multi) { z1 = tz[t0 + 1]; c1 = tc[t0 + 1]; }
-
-#pragma unroll
-    for (int m = 0; m < kBuildM; ++m) {
-        const uint64_t r = rfix + (uint64_t(m) << rs_log2);
-        if (r < row_begin || r >= row_end) continue;
-        const int32_t rank = rank_fixed + ((m & 1) ? s0 : 0) + ((m & 2) ? s1 : 0) +
-                             ((m & 4) ? s2 : 0) + ((m & 8) ? s3 : 0);
-        // term-order fold: first value, then + (accel.rs:191-208)
-        double2 v = signed_coeff(c0, parity_and<kNarrow>(r, z0));
-        if (multi) {
-            const double2 c = signed_coeff(c1, parity_and<kNarrow>(r, z1));
-            v.x = v.x + c.x;
-            v.y = v.y + c.y;
-            for (uint32_t t = t0 + 2; t < t1; ++t) {
-                const double2 ct = signed_coeff(tc[t], parity_and<kNarrow>(r, tz[t]));
-                v.x = v.x + ct.x;
-                v.y = v.y + ct.y;
-            }
-        }
-        const uint64_t pos = (r - row_begin) * G + uint32_t(rank);
-        store_out(r ^ xj, indices + pos);
-        const dvec2 vv = {v.x, v.y};
-        store_out(vv, reinterpret_cast<dvec2 *>(data + pos));
-    }
-}
-
-template <bool kNarrow>
-__global__ __launch_bounds__(kBuildThreads) void k_csr_build(
-    uint64_t row_begin, uint64_t row_end, uint32_t G, uint32_t jw_log2, uint32_t nchunks,
-    const uint64_t *__restrict__ gx, const uint32_t *__restrict__ gterm,
-    const uint64_t *__restrict__ tz, const double2 *__restrict__ tc,
-    const uint32_t *__restrict__ rl_ptr, const RankEntry *__restrict__ rl,
-    uint64_t *__restrict__ indices, double2 *__restrict__ data) {
-    const uint32_t rs_log2 = 8u - jw_log2;
-    const uint32_t tile_log2 = rs_log2 + 4u;
-    const uint32_t lane = threadIdx.x & ((1u << jw_log2) - 1u);
-    const uint32_t ro = threadIdx.x >> jw_log2;
-#if QR_BUILD_LOOP_CHUNKS
-    const uint64_t tile = blockIdx.x;
-    const uint64_t rfix = (((row_begin >> tile_log2) + tile) << tile_log2) | ro;
-    for (uint32_t chunk = 0; chunk < nchunks; ++chunk) {
-        const uint32_t j = (chunk << jw_log2) + lane;
-        if (j >= G) break;
-        build_group_tile<kNarrow>(j, rfix, rs_log2, row_begin, row_end, G, gx, gterm, tz, tc,
-                                  rl_ptr, rl, indices, data);
-    }
-#else
-    const uint32_t chunk = blockIdx.x % nchunks;
-    const uint64_t tile = blockIdx.x / nchunks;
-    const uint32_t j = (chunk << jw_log2) + lane;
-    if (j >= G) return;
-    const uint64_t rfix = (((row_begin >> tile_log2) + tile) << tile_log2) | ro;
-    build_group_tile<kNarrow>(j, rfix, rs_log2, row_begin, row_end, G, gx, gterm, tz, tc, rl_ptr,
-                              rl, indices, data);
-#endif
-}
-
-// LDS-staged build (used when G <= kStageMaxGLean): one block per tile of the
-// shared tile builder (qr_tile.h), without the fused matvec.
+// LDS-staged build: one block per tile of the shared tile builder
+// (qr_tile.h), without the fused matvec.
 template <bool kNarrow, int kChunks, int kMode>
 __global__ __launch_bounds__(kStageThreads) void k_csr_build_staged(TileOp op) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -203,51 +92,29 @@ int launch_csr_build(const qr_op *op, const DeviceTables *t, uint64_t row_begin,
         }
         return QR_OK;
     }
-    if (G <= kStageMaxGLean) {
-        const uint32_t jw = stage_jw_log2(G);
-        const uint32_t nch = (G + (1u << jw) - 1) >> jw;
-        const uint32_t tl2 = 10 - jw + 4;
-        const uint64_t nt = ((row_end - 1) >> tl2) - (row_begin >> tl2) + 1;
-        if (nt > 0x7FFFFFFFull) return fail(QR_ERR_SIZE, "CSR build grid too large");
-        uint32_t dbuf = 0;
-        const uint32_t nh = heavy_fit(t->nheavy, jw);
-        const size_t shmem = stage_bytes(G, jw, &dbuf) + heavy_bytes(nh, jw);
-        TileOp top{t->gx, t->gterm, t->tz, t->tc, t->rl_ptr, t->rl, d_indices,
-                   reinterpret_cast<double2 *>(d_data), nullptr, nullptr, nullptr,
-                   row_begin, row_end, G, jw, nch, dbuf, t->heavy, nh, 0, 0, G};
-        auto pick = [&](auto narrow_t, auto heavy_t) {
-            constexpr bool kN = decltype(narrow_t)::value;
-            constexpr int kH = decltype(heavy_t)::value ? kTileHeavy : kTilePlain;
-            return nch <= 1 ? k_csr_build_staged<kN, 1, kH>
-                   : nch <= 2 ? k_csr_build_staged<kN, 2, kH>
-                   : nch <= uint32_t(kStageMaxChunks) ? k_csr_build_staged<kN, kStageMaxChunks, kH>
-                                                      : k_csr_build_staged<kN, kStageLeanChunks, kH>;
-        };
-        using T = std::true_type;
-        using F = std::false_type;
-        auto kst = narrow ? (nh ? pick(T{}, T{}) : pick(T{}, F{}))
-                          : (nh ? pick(F{}, T{}) : pick(F{}, F{}));
-        QR_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(kst),
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem)));
-        hipLaunchKernelGGL(kst, dim3(unsigned(nt)), dim3(kStageThreads), shmem, s, top);
-        QR_HIP_CHECK(hipGetLastError());
-        return QR_OK;
-    }
-    uint32_t jw_log2 = 0;
-    while ((1u << jw_log2) < G && jw_log2 < 8) ++jw_log2;
-    const uint32_t nchunks = (G + (1u << jw_log2) - 1) >> jw_log2;
-    const uint32_t tile_log2 = 8 - jw_log2 + 4;
-    const uint64_t ntiles = ((row_end - 1) >> tile_log2) - (row_begin >> tile_log2) + 1;
-#if QR_BUILD_LOOP_CHUNKS
-    const uint64_t nblocks = ntiles;
-#else
-    const uint64_t nblocks = ntiles * nchunks;
-#endif
-    if (nblocks > 0x7FFFFFFFull) return fail(QR_ERR_SIZE, "CSR build grid too large");
-    auto kern = narrow ? k_csr_build<true> : k_csr_build<false>;
-    hipLaunchKernelGGL(kern, dim3(unsigned(nblocks)), dim3(kBuildThreads), 0, s, row_begin, row_end,
-                       G, jw_log2, nchunks, t->gx, t->gterm, t->tz, t->tc, t->rl_ptr, t->rl,
-                       d_indices, reinterpret_cast<double2 *>(d_data));
+    // G <= kSubMaxG: one or two group chunks per thread
+    const uint32_t jw = stage_jw_log2(G);
+    const uint32_t nch = (G + (1u << jw) - 1) >> jw;
+    const uint32_t tl2 = 10 - jw + 4;
+    const uint64_t nt = ((row_end - 1) >> tl2) - (row_begin >> tl2) + 1;
+    if (nt > 0x7FFFFFFFull) return fail(QR_ERR_SIZE, "CSR build grid too large");
+    uint32_t dbuf = 0;
+    const uint32_t nh = heavy_fit(t->nheavy, jw);
+    const size_t shmem = stage_bytes(G, jw, &dbuf) + heavy_bytes(nh, jw);
+    TileOp top{t->gx, t->gterm, t->tz, t->tc, t->rl_ptr, t->rl, d_indices,
+               reinterpret_cast<double2 *>(d_data), nullptr, nullptr, nullptr,
+               row_begin, row_end, G, jw, nch, dbuf, t->heavy, nh, 0, 0, G};
+    auto pick = [&](auto narrow_t, auto heavy_t) {
+        constexpr bool kN = decltype(narrow_t)::value;
+        constexpr int kH = decltype(heavy_t)::value ? kTileHeavy : kTilePlain;
+        return nch <= 1 ? k_csr_build_staged<kN, 1, kH> : k_csr_build_staged<kN, 2, kH>;
+    };
+    using T = std::true_type;
+    using F = std::false_type;
+    auto kst = narrow ? (nh ? pick(T{}, T{}) : pick(T{}, F{})) : (nh ? pick(F{}, T{}) : pick(F{}, F{}));
+    QR_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(kst),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem)));
+    hipLaunchKernelGGL(kst, dim3(unsigned(nt)), dim3(kStageThreads), shmem, s, top);
     QR_HIP_CHECK(hipGetLastError());
     return QR_OK;
 }
