@@ -131,14 +131,32 @@ __global__ __launch_bounds__(256) void k_batch_mv(const BatchMvDesc *__restrict_
                         acc.y = fma(a[u].y, xb.x, acc.y);
                     }
                 }
-                for (; k < k1; k += 8) {
-                    const uint64_t c = __builtin_nontemporal_load(d.indices + k);
-                    const dvec2 av = __builtin_nontemporal_load(reinterpret_cast<const dvec2 *>(d.data + k));
-                    const double2 xb = d.x[c];
-                    acc.x = fma(av.x, xb.x, acc.x);
-                    acc.x = fma(-av.y, xb.y, acc.x);
-                    acc.y = fma(av.x, xb.y, acc.y);
-                    acc.y = fma(av.y, xb.x, acc.y);
+                if (k < k1) {
+                    // the fewer than 8 entries left (all of a short row):
+                    // predicated, every load issued before the first is used
+                    uint64_t c[8];
+                    double2 a[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        c[u] = 0;
+                        a[u] = make_double2(0.0, 0.0);
+                        if (k + 8 * u < k1) {
+                            c[u] = __builtin_nontemporal_load(d.indices + k + 8 * u);
+                            const dvec2 av = __builtin_nontemporal_load(
+                                reinterpret_cast<const dvec2 *>(d.data + k + 8 * u));
+                            a[u] = make_double2(av.x, av.y);
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        if (k + 8 * u < k1) {
+                            const double2 xb = d.x[c[u]];
+                            acc.x = fma(a[u].x, xb.x, acc.x);
+                            acc.x = fma(-a[u].y, xb.y, acc.x);
+                            acc.y = fma(a[u].x, xb.y, acc.y);
+                            acc.y = fma(a[u].y, xb.x, acc.y);
+                        }
+                    }
                 }
             }
 #pragma unroll
