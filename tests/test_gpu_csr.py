@@ -122,8 +122,9 @@ def test_kron_modes_match_dense():
 
 def test_build_rows_slices_and_wide_masks():
     """qr_csr_build_rows on row slices, including 33- to 40-qubit operators (64-bit
-    masks), for every kernel class: 1, 2, 4 and 6 group chunks per thread
-    (G <= 1024, 2048, 4096, 6144) and the unstaged kernel (G > 6144)."""
+    masks), for every build route: 1 and 2 group chunks per thread (G <= 1024,
+    2048) and sub-operators (G > 2048: two to five subtrees of the mask trie,
+    with 32- and 64-bit masks)."""
     import ctypes
     import qrusty_amd as qa
     from qrusty_amd import _lib as L
