@@ -50,6 +50,18 @@ const char *qr_last_error(void);
 int qr_abi_version(void);
 
 /* ------------------------------------------------------------------------
+ * Device memory for callers without a HIP binding of their own (the qrusty
+ * side holds Vec buffers; these give it the device counterparts, the role of
+ * the qr_alloc / qr_free pair of the boundary).  Buffers belong to the current
+ * device.  qr_copy_to_host synchronises the stream; qr_copy_to_device is
+ * stream-ordered and synchronises too when `src` is pageable host memory.
+ * ---------------------------------------------------------------------- */
+int qr_device_alloc(uint64_t bytes, void **d_out);
+int qr_device_free(void *d_ptr);
+int qr_copy_to_device(void *d_dst, const void *src, uint64_t bytes, void *stream);
+int qr_copy_to_host(void *dst, const void *d_src, uint64_t bytes, void *stream);
+
+/* ------------------------------------------------------------------------
  * Pauli labels.  Replaces Pauli::new / parse_label / x_indices / z_indices /
  * phase (qrusty/src/lib.rs:125-181): label ^([+-]?)1?([ij]?)([IXYZ]+)$, qubit
  * q is label[len-1-q]; x bit = X or Y, z bit = Y or Z;

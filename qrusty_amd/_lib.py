@@ -31,6 +31,10 @@ f64 = ctypes.c_double
 SIGNATURES = {
     "qr_last_error": (ctypes.c_char_p, []),
     "qr_abi_version": (ctypes.c_int, []),
+    "qr_device_alloc": (ctypes.c_int, [u64, ctypes.POINTER(vp)]),
+    "qr_device_free": (ctypes.c_int, [vp]),
+    "qr_copy_to_device": (ctypes.c_int, [vp, vp, u64, vp]),
+    "qr_copy_to_host": (ctypes.c_int, [vp, vp, u64, vp]),
     "qr_pauli_parse": (ctypes.c_int, [ctypes.c_char_p, u32p, u32p, u32p, u64p, u64p]),
     "qr_op_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_char_p), f64p, u64, i32,
                                     ctypes.POINTER(vp)]),

@@ -280,6 +280,36 @@ const char *qr_last_error(void) { return g_last_error.c_str(); }
 
 int qr_abi_version(void) { return QRHIP_ABI_VERSION; }
 
+int qr_device_alloc(uint64_t bytes, void **d_out) {
+    if (!d_out) return fail(QR_ERR_ARG, "null argument");
+    *d_out = nullptr;
+    if (bytes == 0) return QR_OK;
+    const hipError_t e = hipMalloc(d_out, bytes);
+    if (e == hipErrorOutOfMemory) return fail(QR_ERR_NOMEM, "device out of memory");
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+    return QR_OK;
+}
+
+int qr_device_free(void *d_ptr) {
+    if (d_ptr) QR_HIP_CHECK(hipFree(d_ptr));
+    return QR_OK;
+}
+
+int qr_copy_to_device(void *d_dst, const void *src, uint64_t bytes, void *stream) {
+    if (bytes == 0) return QR_OK;
+    if (!d_dst || !src) return fail(QR_ERR_ARG, "null argument");
+    QR_HIP_CHECK(hipMemcpyAsync(d_dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+    return QR_OK;
+}
+
+int qr_copy_to_host(void *dst, const void *d_src, uint64_t bytes, void *stream) {
+    if (bytes == 0) return QR_OK;
+    if (!dst || !d_src) return fail(QR_ERR_ARG, "null argument");
+    QR_HIP_CHECK(hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+    QR_HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
+    return QR_OK;
+}
+
 int qr_pauli_parse(const char *label, uint32_t *num_qubits, uint32_t *base_phase,
                    uint32_t *phase, uint64_t *x_mask, uint64_t *z_mask) {
     uint32_t nq, bp, ny;
