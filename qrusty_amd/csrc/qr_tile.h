@@ -15,9 +15,9 @@
 // bit-identical to the reference.
 //
 // With kMatvec the tile also produces y[r] = sum_g value * x[r ^ x_g] for its
-// rows from the values still in registers (FP64 FMAs, a fixed reduction tree
-// over the JW lanes of a row), i.e. a CSR matvec fused into the build that
-// never reads the CSR back.
+// rows from the values still in registers (FP64 FMAs per thread, then a fixed
+// reduction tree over the JW lanes of a row), i.e. a CSR matvec fused into the
+// build that never reads the CSR back; without CSR outputs nothing is staged.
 #pragma once
 
 #include "qr_internal.h"
@@ -267,6 +267,11 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
         const int32_t off = kSub ? off_fix + ((m & 1) ? o0 : 0) + ((m & 2) ? o1 : 0) +
                                        ((m & 4) ? o2 : 0) + ((m & 8) ? o3 : 0)
                                  : 0;
+        // fused matvec: this thread's entries of row r, from registers.  Its
+        // groups are the same in every batch, so across the tile's batches it
+        // gathers x[r ^ x_j] for rows r that differ in the batch bits only:
+        // the same few cache lines.
+        double2 acc = make_double2(0.0, 0.0);
 #pragma unroll
         for (int c = 0; c < kChunks; ++c) {
             const uint32_t j = (uint32_t(c) << jw_log2) + lane;
@@ -297,9 +302,18 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
                 }
             }
             const uint64_t col = r ^ uint64_t(xj[c]);
-            const uint32_t slot = (ro * G) + uint32_t(rank - off);
-            si[slot] = col;
-            sd[slot] = v;
+            if (kMatvec) {
+                const double2 xb = op.x[col];
+                acc.x = fma(v.x, xb.x, acc.x);
+                acc.x = fma(-v.y, xb.y, acc.x);
+                acc.y = fma(v.x, xb.y, acc.y);
+                acc.y = fma(v.y, xb.x, acc.y);
+            }
+            if (!kMatvec || op.indices) {
+                const uint32_t slot = (ro * G) + uint32_t(rank - off);
+                si[slot] = col;
+                sd[slot] = v;
+            }
         }
         __syncthreads();
         const uint64_t b0 = tile_base + (uint64_t(m) << rs_log2);
@@ -307,21 +321,8 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
         const uint64_t b1 = b0 + (uint64_t(1) << rs_log2);
         const uint64_t hi_r = b1 < row_end ? b1 : row_end;
         if (kMatvec) {
-            // y of row ro from the staged entries, in column order: lane takes
-            // positions lane, lane + JW, ...; then a fixed tree over the lanes
-            double2 acc = make_double2(0.0, 0.0);
+            // y of row ro: a fixed tree over the row's lanes
             const uint64_t rr = b0 + ro;
-            if (rr >= row_begin && rr < row_end) {
-                for (uint32_t i = lane; i < G; i += (1u << jw_log2)) {
-                    const uint64_t col = si[ro * G + i];
-                    const double2 v = sd[ro * G + i];
-                    const double2 xb = op.x[col];
-                    acc.x = fma(v.x, xb.x, acc.x);
-                    acc.x = fma(-v.y, xb.y, acc.x);
-                    acc.y = fma(v.x, xb.y, acc.y);
-                    acc.y = fma(v.y, xb.x, acc.y);
-                }
-            }
             const uint32_t wl = jw_log2 < 6 ? jw_log2 : 6;
             for (uint32_t off = (1u << wl) >> 1; off > 0; off >>= 1) {
                 acc.x += __shfl_xor(acc.x, int(off), 64);
