@@ -99,7 +99,8 @@ static void build_rank_lists(const std::vector<uint64_t> &gx, std::vector<uint32
 }
 
 static void free_tables(DeviceTables &t) {
-    void *ptrs[] = {t.gx, t.gterm, t.tz, t.tc, t.rl_ptr, t.rl, t.gkind, t.heavy, t.partials};
+    void *ptrs[] = {t.gx, t.gterm, t.tz, t.tc, t.rl_ptr, t.rl, t.gkind, t.heavy, t.leaf_heavy,
+                    t.partials};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &pt : t.pat) {
@@ -231,7 +232,8 @@ int device_tables(const qr_op *op, DeviceTables **out) {
             (rc = upload(&t.tz, op->tz)) || (rc = upload(&t.tc, tc2)) ||
             (rc = upload(&t.rl_ptr, op->rl_ptr)) || (rc = upload(&t.rl, op->rl)) ||
             (rc = upload(&t.gkind, op->gkind)) ||
-            (!op->heavy.empty() && (rc = upload(&t.heavy, op->heavy)))) {
+            (!op->heavy.empty() && (rc = upload(&t.heavy, op->heavy))) ||
+            (!op->leaf_heavy.empty() && (rc = upload(&t.leaf_heavy, op->leaf_heavy)))) {
             free_tables(t);
             return rc;
         }
@@ -396,7 +398,7 @@ int qr_op_create(const char *const *labels, const double *coeffs, uint64_t num_t
             const Node nd = stack.back();
             stack.pop_back();
             if (nd.hi - nd.lo <= kSubMaxG) {
-                op->leaves.push_back({nd.lo, nd.hi, nd.depth});
+                op->leaves.push_back({nd.lo, nd.hi, nd.depth, 0, 0});
                 continue;
             }
             const uint64_t bit = uint64_t(1) << (63 - __builtin_clzll(op->gx[nd.lo] ^ op->gx[nd.hi - 1]));
@@ -422,6 +424,13 @@ int qr_op_create(const char *const *labels, const double *coeffs, uint64_t num_t
         if (cand.size() > kMaxHeavy) cand.resize(kMaxHeavy);
         for (const auto &c : cand) op->heavy.push_back(c.second);
         std::sort(op->heavy.begin(), op->heavy.end());
+        // each sub-operator's heavy groups, in its own (leaf-local) indices
+        for (auto &lf : op->leaves) {
+            lf.hoff = uint32_t(op->leaf_heavy.size());
+            for (uint32_t g : op->heavy)
+                if (g >= lf.s0 && g < lf.s1) op->leaf_heavy.push_back(g - lf.s0);
+            lf.hcnt = uint32_t(op->leaf_heavy.size()) - lf.hoff;
+        }
     }
     *out = op;
     return QR_OK;

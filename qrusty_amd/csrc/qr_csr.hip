@@ -41,10 +41,10 @@ __global__ __launch_bounds__(kStageThreads) void k_csr_build_staged(TileOp op) {
 // Sub-operators of at most 32 qubits (at most 2048 groups, a 48 KB stage):
 // held to 64 VGPRs, i.e. 8 waves per SIMD, so that two 1024-thread blocks
 // share a CU
-template <bool kNarrow, int kChunks>
+template <bool kNarrow, int kChunks, int kMode>
 __global__ __launch_bounds__(kStageThreads, 8) void k_csr_build_sub(TileOp op) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    build_tile_staged<kNarrow, kChunks, false, kTileSub>(op, blockIdx.x, smem_raw);
+    build_tile_staged<kNarrow, kChunks, false, kMode>(op, blockIdx.x, smem_raw);
 }
 
 __global__ __launch_bounds__(256) void k_indptr(uint64_t row_begin, uint64_t nrows, uint64_t G,
@@ -77,14 +77,22 @@ int launch_csr_build(const qr_op *op, const DeviceTables *t, uint64_t row_begin,
             const auto &lf = op->leaves[k];
             const uint32_t g = lf.s1 - lf.s0, nch = (g + 1023) >> 10;
             uint32_t dbuf = 0;
-            const size_t shmem = stage_bytes(g, jw, &dbuf);
+            const uint32_t nh = heavy_fit(lf.hcnt, jw);
+            const size_t shmem = stage_bytes(g, jw, &dbuf) + heavy_bytes(nh, jw);
             TileOp top{t->gx + lf.s0, t->gterm + lf.s0, t->tz, t->tc, t->rl_ptr + lf.s0, t->rl,
                        d_indices, reinterpret_cast<double2 *>(d_data), nullptr, nullptr, nullptr,
-                       row_begin, row_end, g, jw, nch, dbuf, nullptr, 0, lf.s0, lf.depth, G};
-            // (wide masks do not fit 64 VGPRs without spills: one block per CU)
-            auto kst = narrow ? (nch <= 1 ? k_csr_build_sub<true, 1> : k_csr_build_sub<true, 2>)
-                              : (nch <= 1 ? k_csr_build_staged<false, 1, kTileSub>
-                                          : k_csr_build_staged<false, 2, kTileSub>);
+                       row_begin, row_end, g, jw, nch, dbuf,
+                       nh ? t->leaf_heavy + lf.hoff : nullptr, nh, lf.s0, lf.depth, G};
+            // (wide masks, and heavy groups, do not fit 64 VGPRs without
+            // spills: one block per CU)
+            auto kst = nh ? (narrow ? (nch <= 1 ? k_csr_build_staged<true, 1, kTileSubHeavy>
+                                                : k_csr_build_staged<true, 2, kTileSubHeavy>)
+                                    : (nch <= 1 ? k_csr_build_staged<false, 1, kTileSubHeavy>
+                                                : k_csr_build_staged<false, 2, kTileSubHeavy>))
+                          : (narrow ? (nch <= 1 ? k_csr_build_sub<true, 1, kTileSub>
+                                                : k_csr_build_sub<true, 2, kTileSub>)
+                                    : (nch <= 1 ? k_csr_build_staged<false, 1, kTileSub>
+                                                : k_csr_build_staged<false, 2, kTileSub>));
             QR_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(kst),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem)));
             hipLaunchKernelGGL(kst, dim3(unsigned(nt)), dim3(kStageThreads), shmem, s, top);

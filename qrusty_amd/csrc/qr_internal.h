@@ -84,6 +84,7 @@ struct DeviceTables {
     uint8_t *gkind = nullptr;    // [G]   0 complex, 1 real, 2 imaginary coefficients
     uint32_t *heavy = nullptr;   // [nheavy] heavy groups (qr_op::heavy)
     uint32_t nheavy = 0;
+    uint32_t *leaf_heavy = nullptr;  // qr_op::leaf_heavy
     double *partials = nullptr;  // workspace for <x|y> block partials
     size_t partials_cap = 0;     // in doubles
     // matrix-free apply tables, one set per precision (tile size)
@@ -141,8 +142,10 @@ struct qr_op {
     // per subtree (qr_tile.h TileOp::gbase)
     struct SubLeaf {
         uint32_t s0, s1, depth;
+        uint32_t hoff, hcnt;  // its heavy groups: leaf_heavy[hoff, hoff + hcnt)
     };
     std::vector<SubLeaf> leaves;
+    std::vector<uint32_t> leaf_heavy;  // heavy groups of each leaf, leaf-local
     // device state
     mutable std::mutex mu;
     mutable qr::DeviceTables dev[qr::kMaxDevices];

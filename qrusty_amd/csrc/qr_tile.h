@@ -127,14 +127,16 @@ __host__ __device__ inline uint32_t stage_jw_log2(uint32_t G) {
 }
 
 // kMode: kTilePlain; kTileHeavy, the operator has heavy groups
-// (op.nheavy > 0); kTileSub, a sub-operator (op.nanc, op.gbase, op.gstride).
+// (op.nheavy > 0); kTileSub, a sub-operator (op.nanc, op.gbase, op.gstride);
+// kTileSubHeavy, a sub-operator with heavy groups (leaf-local indices).
 // Separate instantiations, so that plain operators run the plain loop (its
 // register count decides whether two blocks share a CU).
-constexpr int kTilePlain = 0, kTileHeavy = 1, kTileSub = 2;
+constexpr int kTilePlain = 0, kTileHeavy = 1, kTileSub = 2, kTileSubHeavy = 3;
 template <bool kNarrow, int kChunks, bool kMatvec, int kMode = kTilePlain>
 __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t tile,
                                                   unsigned char *smem_raw) {
-    constexpr bool kHeavy = kMode == kTileHeavy, kSub = kMode == kTileSub;
+    constexpr bool kHeavy = kMode == kTileHeavy || kMode == kTileSubHeavy;
+    constexpr bool kSub = kMode == kTileSub || kMode == kTileSubHeavy;
     // the second term of each group stays in registers when few chunks leave
     // room; with more than kStageMaxChunks not even the first one does
     constexpr bool kHold2 = kChunks <= 2;

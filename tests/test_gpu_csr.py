@@ -264,3 +264,44 @@ def test_device_tensor_matvec_stays_on_device(hamiltonians):
     ref = qa.spmat_dot_densevec(m, x.cpu().numpy())
     assert np.array_equal(y.cpu().numpy(), ref)
 
+
+
+@pytest.mark.parametrize("n", [14, 34])
+def test_sub_operators_with_heavy_group(n):
+    """More than 2048 groups (sub-operators) plus a diagonal group of 150 Z
+    strings (a heavy group, folded once per row): the CSR equals the oracle's
+    rows bit for bit, with 32- and 64-bit masks."""
+    import qrusty_amd as qa
+    rng = np.random.default_rng(77 + n)
+    labels = _random_labels(rng, n, 3200, prefixes=False)
+    diag = ["".join(rng.choice(["I", "Z"], size=n)) for _ in range(150)]
+    labels = labels + diag
+    coeffs = rng.standard_normal(len(labels)) + 1j * rng.standard_normal(len(labels))
+    p = orc.Params(labels, coeffs)
+    G = len(np.unique(p.x))
+    print(f"[gpu] sub-operators + heavy n={n} G={G}", flush=True)
+    assert G > 2048
+    op = _op(labels, coeffs)
+    N = 1 << n
+    rows = np.unique(np.concatenate([[0, 1, 2, 3, N - 1], rng.integers(0, N, 40)])).astype(np.uint64)
+    if n <= 14:
+        shape, data, indices, indptr = _export(op.to_matrix_mode("Hip"))
+        ip, ind, dat = orc.build_csr(p)
+        assert indptr.tobytes() == ip.tobytes() and indices.tobytes() == ind.tobytes()
+        assert data.view(np.uint64).tobytes() == dat.view(np.uint64).tobytes()
+    else:
+        import ctypes
+        from qrusty_amd import _lib as L
+        h = op._handle(0)
+        ix = torch.empty(G * 16, dtype=torch.int64, device="cuda")
+        dv = torch.empty(G * 16, dtype=torch.complex128, device="cuda")
+        st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        for r in rows:
+            r0 = int(r) & ~15
+            L.check(L.lib.qr_csr_build_rows(h.h, r0, r0 + 16, ctypes.c_void_p(ix.data_ptr()),
+                                            ctypes.c_void_p(dv.data_ptr()), st))
+            got_ix = ix.cpu().numpy().view(np.uint64).reshape(16, G)[int(r) - r0]
+            got_dv = dv.cpu().numpy().reshape(16, G)[int(r) - r0]
+            cols, vals = orc.make_row(p, int(r))
+            assert np.array_equal(got_ix, cols), f"row {r}"
+            assert np.array_equal(got_dv.view(np.uint64), vals.view(np.uint64)), f"row {r}"
