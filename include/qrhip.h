@@ -278,6 +278,11 @@ int qr_mm_read_close(qr_mm_reader *reader);
  * <x_v | y_v> = sum_r conj(x_v[r]) y_v[r] in FP64.
  * d_y may be NULL when d_expect is given: then only <x|H|x> is produced and
  * y is never written.
+ * Two kernel forms give the same results within rounding (the order of the
+ * sum over groups differs): one gathers partner tiles per group, the other
+ * loads each partner tile into LDS once for all the groups that read it.
+ * The library picks one from the operator's masks and the state size; the
+ * environment variable QR_APPLY_FORM=1 (gather) or 2 (staged) forces it.
  * ---------------------------------------------------------------------- */
 enum qr_precision { QR_PREC_FP64 = 0, QR_PREC_MIXED = 1 };
 

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cstring>
 #include <numeric>
+#include <utility>
 
 namespace qr {
 
@@ -104,7 +105,7 @@ static void free_tables(DeviceTables &t) {
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &pt : t.pat) {
-        void *pp[] = {pt.grp, pt.chunk, pt.ug, pt.pzl, pt.ptptr, pt.ptz, pt.ptc};
+        void *pp[] = {pt.grp, pt.chunk, pt.run, pt.ug, pt.pzl, pt.ptptr, pt.ptz, pt.ptc};
         for (void *p : pp)
             if (p) (void)hipFree(p);
     }
@@ -119,7 +120,7 @@ static void free_tables(DeviceTables &t) {
 // lies in the thread bits costs once per thread, one whose zl lies in the
 // row bits once per block (a small per-tile table), and only the straddling
 // ones cost per row.  Terms keep input order inside a pattern.
-void build_patterns(const qr_op *op, int L, int tb, uint32_t pcap, uint32_t ucap,
+void build_patterns(const qr_op *op, int L, int tb, uint32_t pcap, uint32_t ucap, bool staged,
                     HostPatterns &out) {
     const uint64_t mask = L >= 64 ? ~uint64_t(0) : ((uint64_t(1) << L) - 1);
     const uint64_t tmask = (uint64_t(1) << tb) - 1;
@@ -127,24 +128,54 @@ void build_patterns(const qr_op *op, int L, int tb, uint32_t pcap, uint32_t ucap
     out.ptptr.push_back(0);
     ApplyChunk cur{};
     const uint64_t far_mask = ~mask;
-    // closes the chunk: its groups are reordered by class (patterns stay where
-    // they are; the table rows follow their groups)
-    auto close_chunk = [&out, &cur, far_mask]() {
+    // closes the chunk: its groups are reordered by class, or (staged) by
+    // partner tile and then class (patterns stay where they are; the table
+    // rows follow their groups)
+    auto close_chunk = [&out, &cur, far_mask, staged]() {
         const uint32_t g0 = cur.cg[0], g1 = uint32_t(out.grp.size());
         cur.p1 = uint32_t(out.pzl.size());
         cur.u1 = uint32_t(out.ug.size());
+        cur.r0 = cur.r1 = uint32_t(out.run.size());
         if (g1 > g0) {
             auto cls = [far_mask](const ApplyGroup &a) {
                 return ((a.m & far_mask) ? 1u : 0u) | ((a.u >= 0 || a.ps != a.pk) ? 2u : 0u) |
                        ((a.flags & kGroupReal) ? 0u : 4u);
             };
-            std::stable_sort(out.grp.begin() + g0, out.grp.end(),
-                             [&cls](const ApplyGroup &a, const ApplyGroup &b) { return cls(a) < cls(b); });
-            uint32_t c = 0;
-            for (uint32_t g = g0; g <= g1; ++g) {
-                const uint32_t cg = g < g1 ? cls(out.grp[g]) : uint32_t(kApplyClasses);
-                while (c <= cg && c <= uint32_t(kApplyClasses)) cur.cg[c++] = g;
-                if (g < g1 && out.grp[g].u >= 0) out.ug[cur.u0 + uint32_t(out.grp[g].u)] = g;
+            if (!staged) {
+                std::stable_sort(out.grp.begin() + g0, out.grp.end(),
+                                 [&cls](const ApplyGroup &a, const ApplyGroup &b) { return cls(a) < cls(b); });
+                uint32_t c = 0;
+                for (uint32_t g = g0; g <= g1; ++g) {
+                    const uint32_t cg = g < g1 ? cls(out.grp[g]) : uint32_t(kApplyClasses);
+                    while (c <= cg && c <= uint32_t(kApplyClasses)) cur.cg[c++] = g;
+                }
+            } else {
+                // run class: bit 0 row-varying, bit 1 complex (cls without the far bit)
+                auto key = [&cls, far_mask](const ApplyGroup &a) {
+                    return std::make_pair(a.m & far_mask, cls(a) >> 1);
+                };
+                std::stable_sort(out.grp.begin() + g0, out.grp.end(),
+                                 [&key](const ApplyGroup &a, const ApplyGroup &b) { return key(a) < key(b); });
+                for (uint32_t g = g0; g < g1;) {
+                    ApplyRun r{};
+                    r.hi = out.grp[g].m & far_mask;
+                    uint32_t e = g;
+                    while (e < g1 && (out.grp[e].m & far_mask) == r.hi) ++e;
+                    uint32_t c = 0;
+                    for (uint32_t k = g; k <= e; ++k) {
+                        const uint32_t ck = k < e ? key(out.grp[k]).second : uint32_t(kRunClasses);
+                        while (c <= ck && c <= uint32_t(kRunClasses)) r.g[c++] = k;
+                    }
+                    out.run.push_back(r);
+                    g = e;
+                }
+                cur.r1 = uint32_t(out.run.size());
+                for (uint32_t c = 0; c <= uint32_t(kApplyClasses); ++c) cur.cg[c] = g0;
+                cur.cg[kApplyClasses] = g1;
+            }
+            for (uint32_t g = g0; g < g1; ++g) {
+                if (out.grp[g].u >= 0) out.ug[cur.u0 + uint32_t(out.grp[g].u)] = g;
+                if (out.grp[g].m & far_mask) ++out.nfar;
             }
             out.chunk.push_back(cur);
             out.max_p = std::max(out.max_p, cur.p1 - cur.p0);
@@ -245,20 +276,21 @@ int device_tables(const qr_op *op, DeviceTables **out) {
 }
 
 int device_patterns(const qr_op *op, DeviceTables *t, int slot, int L, int tb, uint32_t pcap,
-                    uint32_t ucap) {
+                    uint32_t ucap, bool staged) {
     std::lock_guard<std::mutex> lock(op->mu);
     DeviceTables::Patterns &pt = t->pat[slot];
     if (pt.L == L) return QR_OK;
-    for (void *p : {(void *)pt.grp, (void *)pt.chunk, (void *)pt.ug, (void *)pt.pzl,
+    for (void *p : {(void *)pt.grp, (void *)pt.chunk, (void *)pt.run, (void *)pt.ug, (void *)pt.pzl,
                     (void *)pt.ptptr, (void *)pt.ptz, (void *)pt.ptc})
         if (p) (void)hipFree(p);
     pt = DeviceTables::Patterns{};
     HostPatterns hp;
-    build_patterns(op, L, tb, pcap, ucap, hp);
+    build_patterns(op, L, tb, pcap, ucap, staged, hp);
     std::vector<double2> c2(hp.ptz.size());
     for (size_t i = 0; i < c2.size(); ++i) c2[i] = make_double2(hp.ptc[2 * i], hp.ptc[2 * i + 1]);
     int rc;
     if ((rc = upload(&pt.grp, hp.grp)) || (rc = upload(&pt.chunk, hp.chunk)) ||
+        (staged && (rc = upload(&pt.run, hp.run))) ||
         (rc = upload(&pt.ug, hp.ug)) || (rc = upload(&pt.pzl, hp.pzl)) ||
         (rc = upload(&pt.ptptr, hp.ptptr)) || (rc = upload(&pt.ptz, hp.ptz)) ||
         (rc = upload(&pt.ptc, c2)))
@@ -266,6 +298,8 @@ int device_patterns(const qr_op *op, DeviceTables *t, int slot, int L, int tb, u
     pt.P = uint32_t(hp.pzl.size());
     pt.ngrp = uint32_t(hp.grp.size());
     pt.nchunk = uint32_t(hp.chunk.size());
+    pt.nrun = uint32_t(hp.run.size());
+    pt.nfar = hp.nfar;
     pt.max_p = hp.max_p;
     pt.max_u = hp.max_u;
     pt.L = L;

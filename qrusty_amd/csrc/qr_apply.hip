@@ -41,6 +41,7 @@
 // then one pass over the block partials of each vector.
 #include "qr_internal.h"
 
+#include <cstdlib>
 #include <type_traits>
 
 namespace qr {
@@ -62,6 +63,10 @@ struct ApplyShape {
     static constexpr size_t kRest = kApplyMaxLDS - kTile - 1024;
     static constexpr uint32_t kPatChunk = uint32_t(kRest * 5 / 8 / 20) < 2048u ? uint32_t(kRest * 5 / 8 / 20) : 2048u;
     static constexpr uint32_t kTableRows = uint32_t(kRest * 3 / 8 / (R * 16)) - 1;
+    // partner-staged form: two tile buffers, the rest split the same way
+    static constexpr size_t kRestS = kApplyMaxLDS - 2 * kTile - 1024;
+    static constexpr uint32_t kPatChunkS = uint32_t(kRestS * 5 / 8 / 20);
+    static constexpr uint32_t kTableRowsS = uint32_t(kRestS * 3 / 8 / (R * 16)) - 1;
 };
 
 template <bool kNarrow>
@@ -338,6 +343,42 @@ __host__ __device__ inline size_t apply_lds_offsets(uint64_t tile_rows, uint32_t
     return u + size_t(max_u) * R * sizeof(double2);
 }
 
+// Per-chunk LDS tables of a tile: the pattern coefficients
+// sA[p] = A_p = sum_t (-1)^popcount(base & z_t) c'_t with their low z bits
+// sZ[p], then the row-bit table sU[u*R + k] = sum over the row-bit patterns of
+// the group of (-1)^popcount(k & (zl >> thread bits)) * A_p; row zero_row
+// stays zero for groups without such patterns.  Ends with a barrier.
+template <bool kNarrow, int R>
+__device__ __forceinline__ void chunk_tables(const ApplyChunk &ch, uint32_t tid, uint64_t base,
+                                             uint32_t zero_row, const ApplyGroup *__restrict__ grp,
+                                             const uint32_t *__restrict__ ug,
+                                             const uint32_t *__restrict__ pzl,
+                                             const uint32_t *__restrict__ ptptr,
+                                             const uint64_t *__restrict__ ptz,
+                                             const double2 *__restrict__ ptc, double2 *sA,
+                                             uint32_t *sZ, double2 *sU) {
+    for (uint32_t p = tid; p < ch.p1 - ch.p0; p += kApplyThreads) {
+        const uint32_t pa = ch.p0 + p;
+        sA[p] = pattern_coeff<kNarrow>(base, ptptr[pa], ptptr[pa + 1], ptz, ptc);
+        sZ[p] = pzl[pa];
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < (ch.u1 - ch.u0 + 1) * R; i += kApplyThreads) {
+        const uint32_t u = i / R, k = i % R;
+        double s_r = 0.0, s_i = 0.0;
+        if (u < ch.u1 - ch.u0) {
+            const ApplyGroup &gg = grp[ug[ch.u0 + u]];
+            for (uint32_t p = gg.pk; p < gg.p1; ++p) {
+                const double2 a = sA[p];
+                if (__popc(k & (sZ[p] >> kApplyThreadBits)) & 1u) { s_r -= a.x; s_i -= a.y; }
+                else { s_r += a.x; s_i += a.y; }
+            }
+        }
+        sU[(u < ch.u1 - ch.u0 ? u : zero_row) * R + k] = make_double2(s_r, s_i);
+    }
+    __syncthreads();
+}
+
 template <bool kMixed, bool kNarrow, bool kFull>
 __global__ __launch_bounds__(kApplyThreads) void k_apply_tiled(
     uint64_t N, uint32_t ntiles, uint32_t tiles_per_xcd, uint32_t ngrp, uint32_t nchunk,
@@ -394,30 +435,7 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_tiled(
     for (uint32_t c = 0; c < nchunk; ++c) {
         const ApplyChunk ch = chunks[c];
         if (c) __syncthreads();  // the previous chunk's tables are no longer read
-        // per-tile pattern coefficients A_p = sum_t (-1)^popcount(base & z_t) c'_t
-        for (uint32_t p = tid; p < ch.p1 - ch.p0; p += kApplyThreads) {
-            const uint32_t pa = ch.p0 + p;
-            sA[p] = pattern_coeff<kNarrow>(base, ptptr[pa], ptptr[pa + 1], ptz, ptc);
-            sZ[p] = pzl[pa];
-        }
-        __syncthreads();
-        // per-tile row-bit table: sU[u*R + k] = sum over the row-bit patterns of
-        // the group of (-1)^popcount(k & (zl >> thread bits)) * A_p; row
-        // zero_row stays zero for groups without such patterns
-        for (uint32_t i = tid; i < (ch.u1 - ch.u0 + 1) * R; i += kApplyThreads) {
-            const uint32_t u = i / R, k = i % R;
-            double s_r = 0.0, s_i = 0.0;
-            if (u < ch.u1 - ch.u0) {
-                const ApplyGroup &gg = grp[ug[ch.u0 + u]];
-                for (uint32_t p = gg.pk; p < gg.p1; ++p) {
-                    const double2 a = sA[p];
-                    if (__popc(k & (sZ[p] >> kApplyThreadBits)) & 1u) { s_r -= a.x; s_i -= a.y; }
-                    else { s_r += a.x; s_i += a.y; }
-                }
-            }
-            sU[(u < ch.u1 - ch.u0 ? u : zero_row) * R + k] = make_double2(s_r, s_i);
-        }
-        __syncthreads();
+        chunk_tables<kNarrow, R>(ch, tid, base, zero_row, grp, ug, pzl, ptptr, ptz, ptc, sA, sZ, sU);
         // one loop per group class, each with a single straight path, in
         // blocks of at most kCompBlock groups flushed after each block
         auto run_class = [&](auto far_t, auto rowvar_t, auto cplx_t, uint32_t g0, uint32_t g1) {
@@ -459,7 +477,6 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_tiled(
     }
 
     if (kMixed && since_flush) flush();
-
     double er = 0.0, ei = 0.0;
     cT *__restrict__ yv = yout ? reinterpret_cast<cT *>(yout) + uint64_t(v) * N : nullptr;
 #pragma unroll
@@ -478,6 +495,184 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_tiled(
         }
         if (partials) {
             const cT xr = xt[l];
+            const double xa = xr.x, xb = xr.y;
+            er = fma(xa, yr, er); er = fma(xb, yi, er);
+            ei = fma(xa, yi, ei); ei = fma(-xb, yr, ei);
+        }
+    }
+    if (partials) {
+        __shared__ double red[kApplyThreads / 64][2];
+        const double a = wave_sum(er), b = wave_sum(ei);
+        if ((tid & 63) == 0) { red[tid / 64][0] = a; red[tid / 64][1] = b; }
+        __syncthreads();
+        if (tid == 0) {
+            double s0 = 0.0, s1 = 0.0;
+            for (int w = 0; w < kApplyThreads / 64; ++w) { s0 += red[w][0]; s1 += red[w][1]; }
+            partials[2 * uint64_t(blockIdx.x)] = s0;
+            partials[2 * uint64_t(blockIdx.x) + 1] = s1;
+        }
+    }
+}
+
+// Partner-staged form of k_apply_tiled (full tiles only).  Instead of one
+// tile of x in LDS and far gathers from global memory, two tile buffers take
+// the partner tiles t ^ hi in turn, one per run of groups sharing hi (hi = 0:
+// the tile itself), so a partner tile is read once per chunk however many
+// groups use it.  The load of run r + 1 (LDS-DMA: global memory straight into
+// LDS, no registers) is in flight while run r is computed.
+constexpr uint32_t kStageTileBytes = 65536;  // one tile: 2^13 complex64 or 2^12 complex128
+
+// One LDS-DMA load: each lane reads the 16 bytes at its `src`, and the wave
+// writes them to LDS at lds + lane * 16 (lds: LDS byte address, wave-uniform,
+// passed in M0; one wait state between the M0 write and the load).  Written
+// as inline asm so that the compiler, which cannot tell the buffer being
+// filled from the one being read, does not wait for the load before every
+// later LDS read; the caller waits for it (vmcnt) and synchronises the block
+// before reading what it wrote.  It has no register result.
+__device__ __forceinline__ void lds_dma16(const void *src, uint32_t lds) {
+    asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off"
+                 :
+                 : "v"(src), "s"(lds)
+                 : "memory", "m0");
+}
+
+typedef __attribute__((address_space(3))) unsigned char lds_byte;
+
+template <bool kMixed, bool kNarrow>
+__global__ __launch_bounds__(kApplyThreads) void k_apply_staged(
+    uint64_t N, uint32_t ntiles, uint32_t tiles_per_xcd, uint32_t nchunk, uint32_t nrun,
+    uint32_t max_p, uint32_t zero_row, const ApplyGroup *__restrict__ grp,
+    const ApplyChunk *__restrict__ chunks, const ApplyRun *__restrict__ runs,
+    const uint32_t *__restrict__ ug, const uint32_t *__restrict__ pzl,
+    const uint32_t *__restrict__ ptptr, const uint64_t *__restrict__ ptz,
+    const double2 *__restrict__ ptc, const void *__restrict__ xin, void *__restrict__ yout,
+    double *__restrict__ partials) {
+    using Shape = ApplyShape<kMixed>;
+    constexpr int L = Shape::L, R = Shape::R;
+    using cT = typename std::conditional<kMixed, float2, double2>::type;
+    constexpr uint64_t tile_rows = uint64_t(1) << L;
+    constexpr uint32_t kLoads = kStageTileBytes / (16u * kApplyThreads);  // 16-byte loads per thread
+    static_assert(tile_rows * sizeof(cT) == kStageTileBytes, "a tile fills one stage buffer");
+    static_assert(kStageTileBytes % (16u * kApplyThreads) == 0, "tile must split into 16-byte loads");
+    __shared__ __align__(16) unsigned char bufs[2 * kStageTileBytes];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t wave = uint32_t(__builtin_amdgcn_readfirstlane(int(tid / 64)));
+    const uint32_t v = blockIdx.x / ntiles;
+    uint32_t tile = blockIdx.x % ntiles;
+    if (tiles_per_xcd) tile = (tile % 8) * tiles_per_xcd + tile / 8;
+    const uint64_t base = uint64_t(tile) * tile_rows;
+    const cT *__restrict__ xv = reinterpret_cast<const cT *>(xin) + uint64_t(v) * N;
+
+    size_t oA, oZ, oU;
+    apply_lds_offsets<cT>(0, max_p, R, 0, &oA, &oZ, &oU);
+    double2 *sA = reinterpret_cast<double2 *>(smem_raw + oA);
+    uint32_t *sZ = reinterpret_cast<uint32_t *>(smem_raw + oZ);
+    double2 *sU = reinterpret_cast<double2 *>(smem_raw + oU);
+
+    // partner tile base ^ hi into buffer b: each wave fills 1 KB per load,
+    // lane i the 16 bytes at i * 16
+    const uint32_t lds_bufs = uint32_t(uintptr_t((lds_byte *)bufs));  // LDS address of bufs
+    auto stage = [&](uint64_t hi, uint32_t b) {
+        const unsigned char *src = reinterpret_cast<const unsigned char *>(xv + (base ^ hi));
+#pragma unroll
+        for (uint32_t j = 0; j < kLoads; ++j) {
+            const uint32_t off = (j * kApplyThreads + wave * 64u) * 16u;
+            lds_dma16(src + off + (tid % 64u) * 16u, lds_bufs + b * kStageTileBytes + off);
+        }
+    };
+
+    double ar[R], ai[R];
+    float sr[R], si[R], cr[R], ci[R], br[R], bi[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        ar[k] = ai[k] = 0.0;
+        sr[k] = si[k] = cr[k] = ci[k] = br[k] = bi[k] = 0.0f;
+    }
+    uint32_t since_flush = 0;
+    auto flush = [&]() {
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            float e;
+            two_sum(sr[k], br[k], sr[k], e);
+            cr[k] += e;
+            two_sum(si[k], bi[k], si[k], e);
+            ci[k] += e;
+            br[k] = bi[k] = 0.0f;
+        }
+    };
+    cT xg[R];
+    // the groups of one run, all gathering from the tile in `xt`
+    auto run_groups = [&](const ApplyRun &run, const cT *xt) {
+        auto run_class = [&](auto rowvar_t, auto cplx_t, uint32_t g0, uint32_t g1) {
+            constexpr bool kRowvar = decltype(rowvar_t)::value;
+            constexpr bool kCplx = decltype(cplx_t)::value;
+            for (uint32_t gb = g0; gb < g1;) {
+                const uint32_t room = uint32_t(kCompBlock) - since_flush;
+                const uint32_t ge = g1 - gb > room ? gb + room : g1;
+                for (uint32_t g = gb; g < ge; ++g) {
+                    const ApplyGroup gd = grp[g];
+                    asm volatile("" ::"s"(gd.p0), "s"(gd.ps), "s"(gd.pk), "s"(gd.p1), "s"(gd.u),
+                                 "s"(gd.flags));
+                    gather<false, true, R>(gd.m, tid, tile_rows, base, xt, xv, xg);
+                    accumulate<kMixed, true, kRowvar, kCplx, R>(gd, tid, tile_rows, zero_row, sA, sZ,
+                                                                sU, xg, ar, ai, br, bi);
+                }
+                since_flush += ge - gb;
+                gb = ge;
+                if (since_flush == uint32_t(kCompBlock)) {
+                    if (kMixed) flush();
+                    since_flush = 0;
+                }
+            }
+        };
+        using F = std::false_type;
+        using T = std::true_type;
+        run_class(F{}, F{}, run.g[0], run.g[1]);
+        run_class(T{}, F{}, run.g[1], run.g[2]);
+        run_class(F{}, T{}, run.g[2], run.g[3]);
+        run_class(T{}, T{}, run.g[3], run.g[4]);
+    };
+    if (nrun) stage(runs[0].hi, 0);
+    uint32_t r = 0;
+    for (uint32_t c = 0; c < nchunk; ++c) {
+        const ApplyChunk ch = chunks[c];
+        if (c) __syncthreads();  // the previous chunk's tables are no longer read
+        chunk_tables<kNarrow, R>(ch, tid, base, zero_row, grp, ug, pzl, ptptr, ptz, ptc, sA, sZ, sU);
+        for (; r < ch.r1; ++r) {
+            // run r's tile has landed (each wave waits for its own loads, the
+            // barrier for all of them), and no wave still reads run r - 1's
+            // buffer, which the next load overwrites
+            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+            __syncthreads();
+            if (r + 1 < nrun) stage(runs[r + 1].hi, (r + 1) & 1u);
+            run_groups(runs[r], reinterpret_cast<const cT *>(bufs + (r & 1u) * kStageTileBytes));
+        }
+    }
+    if (kMixed && since_flush) flush();
+
+    // y, and <x|y> with x of the thread's rows read from global memory (the
+    // buffers hold partner tiles)
+    constexpr bool kFull = true;
+    double er = 0.0, ei = 0.0;
+    cT *__restrict__ yv = yout ? reinterpret_cast<cT *>(yout) + uint64_t(v) * N : nullptr;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const uint64_t l = tid + uint64_t(k) * kApplyThreads;
+        if (!kFull && l >= tile_rows) break;
+        double yr, yi;
+        if (!kMixed) {
+            yr = ar[k];
+            yi = ai[k];
+            if (yv) store_y(reinterpret_cast<double2 *>(yv) + base + l, make_double2(yr, yi));
+        } else {
+            yr = double(sr[k]) + double(cr[k]);
+            yi = double(si[k]) + double(ci[k]);
+            if (yv) store_y(reinterpret_cast<float2 *>(yv) + base + l, make_float2(sr[k] + cr[k], si[k] + ci[k]));
+        }
+        if (partials) {
+            const cT xr = xv[base + l];
             const double xa = xr.x, xb = xr.y;
             er = fma(xa, yr, er); er = fma(xb, yi, er);
             ei = fma(xa, yi, ei); ei = fma(-xb, yr, ei);
@@ -547,14 +742,74 @@ static int launch_tiled(const qr_op *op, DeviceTables *t, uint32_t nvec, const v
     return QR_OK;
 }
 
+template <bool kMixed, bool kNarrow>
+static int launch_staged(const qr_op *op, DeviceTables *t, uint32_t nvec, const void *d_x,
+                         void *d_y, double *partials, uint32_t ntiles, hipStream_t s) {
+    using Shape = ApplyShape<kMixed>;
+    using cT = typename std::conditional<kMixed, float2, double2>::type;
+    const uint64_t N = uint64_t(1) << op->nq;
+    const DeviceTables::Patterns &pt = t->pat[2 + (kMixed ? 1 : 0)];
+    // dynamic LDS holds the tables; the two tile buffers are static
+    const size_t shmem = apply_lds_offsets<cT>(0, pt.max_p, Shape::R, pt.max_u + 1, nullptr,
+                                               nullptr, nullptr);
+    if (shmem + 2 * kStageTileBytes + 1024 > kApplyMaxLDS)
+        return fail(QR_ERR_SIZE, "apply: LDS stage too large");
+    const uint32_t tpx = (ntiles % 8 == 0) ? ntiles / 8 : 0;
+    auto kern = k_apply_staged<kMixed, kNarrow>;
+    QR_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem)));
+    const uint64_t nblocks = uint64_t(ntiles) * nvec;
+    if (nblocks > 0x7FFFFFFFull) return fail(QR_ERR_SIZE, "apply grid too large");
+    hipLaunchKernelGGL(kern, dim3(unsigned(nblocks)), dim3(kApplyThreads), shmem, s, N, ntiles, tpx,
+                       pt.nchunk, pt.nrun, pt.max_p, pt.max_u, pt.grp, pt.chunk, pt.run, pt.ug, pt.pzl,
+                       pt.ptptr, pt.ptz, pt.ptc, d_x, d_y, partials);
+    QR_HIP_CHECK(hipGetLastError());
+    return QR_OK;
+}
+
+// Which form applies the operator.  The partner-staged kernel needs full
+// tiles, and pays off when far groups share partner tiles: it loads a partner
+// tile once per run instead of gathering it once per group, but the load of a
+// one-group run is not hidden behind much work, and while the state is small
+// enough to stay in the caches the per-group gathers are cheap.  Measured
+// (far groups F on D distinct partner offsets, speedup of the staged form):
+//   24-qubit k-local, 100..3000 terms: F/D 1.8..5.7, 1.15..1.58x;
+//   28-qubit k-local, 10000 terms: F/D 4.8, 1.40x (mixed), 1.22x (FP64);
+//   Heisenberg ring, 22 and 24 qubits: F/D 1, 0.85..0.93x;
+//   20-qubit random (8 MB complex64 state): F/D 2.3 -> 0.97x, 7.8 -> 1.18x.
+// Hence: staged when F >= 1.5 D for states of at least 32 MiB per vector,
+// F >= 6 D below that.  The environment variable QR_APPLY_FORM (1 gather,
+// 2 staged) overrides the choice, for tests and measurement.
+static bool prefer_staged(const qr_op *op, int L, size_t elem_bytes) {
+    const char *e = getenv("QR_APPLY_FORM");
+    if (e && atoi(e) == 1) return false;
+    if (e && atoi(e) == 2) return true;
+    // gx ascends, so groups with equal bits above the tile are adjacent
+    uint64_t far = 0, distinct = 0, prev = 0;
+    for (uint64_t g = 0; g < op->G; ++g) {
+        const uint64_t hi = op->gx[g] >> L;
+        if (!hi) continue;
+        ++far;
+        if (distinct == 0 || hi != prev) ++distinct;
+        prev = hi;
+    }
+    if (far == 0) return false;
+    const bool big = (uint64_t(1) << op->nq) * elem_bytes >= (uint64_t(32) << 20);
+    return big ? 2 * far >= 3 * distinct : far >= 6 * distinct;
+}
+
 int launch_pauli_apply(const qr_op *op, DeviceTables *t, int32_t precision, uint32_t nvec,
                        const void *d_x, void *d_y, double *d_expect, hipStream_t s) {
     const bool mixed = precision == QR_PREC_MIXED, narrow = op->nq <= 32;
     const int Lmax = mixed ? ApplyShape<true>::L : ApplyShape<false>::L;
     const int L = int(op->nq) < Lmax ? int(op->nq) : Lmax;
-    const uint32_t pcap = mixed ? ApplyShape<true>::kPatChunk : ApplyShape<false>::kPatChunk;
-    const uint32_t ucap = mixed ? ApplyShape<true>::kTableRows : ApplyShape<false>::kTableRows;
-    int rc = device_patterns(op, t, mixed ? 1 : 0, L, kApplyThreadBits, pcap, ucap);
+    const bool staged = L == Lmax && prefer_staged(op, L, mixed ? 8 : 16);
+    const uint32_t pcap = staged ? (mixed ? ApplyShape<true>::kPatChunkS : ApplyShape<false>::kPatChunkS)
+                                 : (mixed ? ApplyShape<true>::kPatChunk : ApplyShape<false>::kPatChunk);
+    const uint32_t ucap = staged ? (mixed ? ApplyShape<true>::kTableRowsS : ApplyShape<false>::kTableRowsS)
+                                 : (mixed ? ApplyShape<true>::kTableRows : ApplyShape<false>::kTableRows);
+    int rc = device_patterns(op, t, (mixed ? 1 : 0) + (staged ? 2 : 0), L, kApplyThreadBits, pcap,
+                             ucap, staged);
     if (rc) return rc;
     const uint64_t N = uint64_t(1) << op->nq;
     if ((N >> L) * uint64_t(nvec) > 0x7FFFFFFFull || nvec == 0)
@@ -572,8 +827,13 @@ int launch_pauli_apply(const qr_op *op, DeviceTables *t, int32_t precision, uint
         }
         partials = t->partials;
     }
-    if (mixed) rc = narrow ? launch_tiled<true, true>(op, t, nvec, d_x, d_y, partials, ntiles, s)
-                           : launch_tiled<true, false>(op, t, nvec, d_x, d_y, partials, ntiles, s);
+    if (staged) {
+        if (mixed) rc = narrow ? launch_staged<true, true>(op, t, nvec, d_x, d_y, partials, ntiles, s)
+                               : launch_staged<true, false>(op, t, nvec, d_x, d_y, partials, ntiles, s);
+        else rc = narrow ? launch_staged<false, true>(op, t, nvec, d_x, d_y, partials, ntiles, s)
+                         : launch_staged<false, false>(op, t, nvec, d_x, d_y, partials, ntiles, s);
+    } else if (mixed) rc = narrow ? launch_tiled<true, true>(op, t, nvec, d_x, d_y, partials, ntiles, s)
+                                  : launch_tiled<true, false>(op, t, nvec, d_x, d_y, partials, ntiles, s);
     else rc = narrow ? launch_tiled<false, true>(op, t, nvec, d_x, d_y, partials, ntiles, s)
                      : launch_tiled<false, false>(op, t, nvec, d_x, d_y, partials, ntiles, s);
     if (rc) return rc;

@@ -64,12 +64,26 @@ constexpr uint32_t kGroupReal = 1;  // every coefficient of the group is real
 // together.  Inside a chunk the groups are ordered by class -- bit 0: the mask
 // reaches outside the tile ("far"), bit 1: D varies by row (row-bit or
 // straddling patterns), bit 2: complex coefficients -- and class c holds the
-// groups [cg[c], cg[c+1]).
+// groups [cg[c], cg[c+1]).  For the partner-staged kernel the groups are
+// ordered by partner tile instead, in runs [r0, r1) (ApplyRun).
 constexpr int kApplyClasses = 8;
 struct ApplyChunk {
     uint32_t cg[kApplyClasses + 1];
     uint32_t p0, p1, u0, u1;
-    uint32_t pad[3];
+    uint32_t r0, r1;
+    uint32_t pad;
+};
+
+// Partner-staged apply: the groups of a chunk whose masks share the bits
+// above the tile (`hi`, a row offset) gather from the same partner tile,
+// which is loaded into LDS once for all of them.  Inside a run the groups are
+// ordered by class -- bit 0: D varies by row, bit 1: complex coefficients --
+// and class c holds the groups [g[c], g[c+1]).
+constexpr int kRunClasses = 4;
+struct ApplyRun {
+    uint64_t hi;
+    uint32_t g[kRunClasses + 1];
+    uint32_t pad;
 };
 
 // Device copies of an operator's tables (one set per device that used it).
@@ -87,36 +101,41 @@ struct DeviceTables {
     uint32_t *leaf_heavy = nullptr;  // qr_op::leaf_heavy
     double *partials = nullptr;  // workspace for <x|y> block partials
     size_t partials_cap = 0;     // in doubles
-    // matrix-free apply tables, one set per precision (tile size)
+    // matrix-free apply tables, one set per precision (tile size) and form
+    // (slot = precision + 2 * staged)
     struct Patterns {
         int L = -1;
-        uint32_t P = 0, ngrp = 0, nchunk = 0, max_p = 0, max_u = 0;
+        uint32_t P = 0, ngrp = 0, nchunk = 0, nrun = 0, max_p = 0, max_u = 0;
+        uint32_t nfar = 0;  // groups whose mask reaches outside the tile
         ApplyGroup *grp = nullptr;    // [ngrp]
         ApplyChunk *chunk = nullptr;  // [nchunk]
+        ApplyRun *run = nullptr;      // [nrun] (staged form only)
         uint32_t *ug = nullptr;     // [U]   group (index into grp) of each table row
         uint32_t *pzl = nullptr;    // [P]   z & (2^L - 1) of the pattern
         uint32_t *ptptr = nullptr;  // [P+1] term range of each pattern
         uint64_t *ptz = nullptr;    // [T]   z masks in pattern order
         double2 *ptc = nullptr;     // [T]   c' in pattern order
-    } pat[2];
+    } pat[4];
 };
 
 // Host build of the apply tables for tile size L, tb thread-index bits, at
 // most pcap patterns and ucap table rows per chunk (see qr_apply.hip).  A
 // group with more than pcap patterns is split into several ApplyGroups with
-// the same mask.
+// the same mask.  `staged` orders each chunk by partner tile (ApplyRun)
+// instead of by class.
 struct HostPatterns {
     std::vector<ApplyGroup> grp;
     std::vector<ApplyChunk> chunk;
     std::vector<uint32_t> ug, pzl, ptptr;
     std::vector<uint64_t> ptz;
     std::vector<double> ptc;  // 2 per term
-    uint32_t max_p = 0, max_u = 0;
+    std::vector<ApplyRun> run;  // staged form only
+    uint32_t max_p = 0, max_u = 0, nfar = 0;
 };
-void build_patterns(const qr_op *op, int L, int tb, uint32_t pcap, uint32_t ucap,
+void build_patterns(const qr_op *op, int L, int tb, uint32_t pcap, uint32_t ucap, bool staged,
                     HostPatterns &out);
 int device_patterns(const qr_op *op, DeviceTables *t, int slot, int L, int tb, uint32_t pcap,
-                    uint32_t ucap);
+                    uint32_t ucap, bool staged);
 
 }  // namespace qr
 
