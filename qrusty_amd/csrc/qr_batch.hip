@@ -86,6 +86,75 @@ struct BatchMvDesc {
 };
 constexpr uint32_t kMvRows = 1024;
 
+// Rows [r0, r1) of one operator's y = A x, 8 lanes per row; the CSR stream is
+// loaded nontemporally (kNT) for long rows only, as in k_csr_matvec.
+template <bool kNT>
+__device__ __forceinline__ void batch_mv_rows(const BatchMvDesc &d, uint64_t r0, uint64_t r1,
+                                              uint32_t lane) {
+    auto ld = [](const auto *p) {
+        if constexpr (kNT) return __builtin_nontemporal_load(p);
+        else return *p;
+    };
+    for (uint64_t rb = r0; rb < r1; rb += 32) {
+        const uint64_t row = rb + (threadIdx.x >> 3);
+        double2 acc = make_double2(0.0, 0.0);
+        if (row < r1) {
+            const uint64_t k1 = (row + 1) * d.G;
+            uint64_t k = row * d.G + lane;
+            for (; k + 56 < k1; k += 64) {
+                uint64_t c[8];
+                double2 a[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    c[u] = ld(d.indices + k + 8 * u);
+                    const dvec2 av = ld(reinterpret_cast<const dvec2 *>(d.data + k + 8 * u));
+                    a[u] = make_double2(av.x, av.y);
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const double2 xb = d.x[c[u]];
+                    acc.x = fma(a[u].x, xb.x, acc.x);
+                    acc.x = fma(-a[u].y, xb.y, acc.x);
+                    acc.y = fma(a[u].x, xb.y, acc.y);
+                    acc.y = fma(a[u].y, xb.x, acc.y);
+                }
+            }
+            if (k < k1) {
+                // the fewer than 8 entries left (all of a short row):
+                // predicated, every load issued before the first is used
+                uint64_t c[8];
+                double2 a[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    c[u] = 0;
+                    a[u] = make_double2(0.0, 0.0);
+                    if (k + 8 * u < k1) {
+                        c[u] = ld(d.indices + k + 8 * u);
+                        const dvec2 av = ld(reinterpret_cast<const dvec2 *>(d.data + k + 8 * u));
+                        a[u] = make_double2(av.x, av.y);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    if (k + 8 * u < k1) {
+                        const double2 xb = d.x[c[u]];
+                        acc.x = fma(a[u].x, xb.x, acc.x);
+                        acc.x = fma(-a[u].y, xb.y, acc.x);
+                        acc.y = fma(a[u].x, xb.y, acc.y);
+                        acc.y = fma(a[u].y, xb.x, acc.y);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int off = 4; off > 0; off /= 2) {
+            acc.x += __shfl_xor(acc.x, off, 8);
+            acc.y += __shfl_xor(acc.y, off, 8);
+        }
+        if (row < r1 && lane == 0) d.y[row] = acc;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_batch_mv(const BatchMvDesc *__restrict__ ops, uint32_t nops,
                                                   uint64_t total_items,
                                                   unsigned long long *__restrict__ counter) {
@@ -106,66 +175,8 @@ __global__ __launch_bounds__(256) void k_batch_mv(const BatchMvDesc *__restrict_
         const BatchMvDesc &d = ops[lo];
         const uint64_t r0 = (item - d.item_begin) * kMvRows;
         const uint64_t r1 = r0 + kMvRows < d.rows ? r0 + kMvRows : d.rows;
-        for (uint64_t rb = r0; rb < r1; rb += 32) {
-            const uint64_t row = rb + (threadIdx.x >> 3);
-            double2 acc = make_double2(0.0, 0.0);
-            if (row < r1) {
-                const uint64_t k1 = (row + 1) * d.G;
-                uint64_t k = row * d.G + lane;
-                for (; k + 56 < k1; k += 64) {
-                    uint64_t c[8];
-                    double2 a[8];
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) {
-                        c[u] = __builtin_nontemporal_load(d.indices + k + 8 * u);
-                        const dvec2 av = __builtin_nontemporal_load(
-                            reinterpret_cast<const dvec2 *>(d.data + k + 8 * u));
-                        a[u] = make_double2(av.x, av.y);
-                    }
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) {
-                        const double2 xb = d.x[c[u]];
-                        acc.x = fma(a[u].x, xb.x, acc.x);
-                        acc.x = fma(-a[u].y, xb.y, acc.x);
-                        acc.y = fma(a[u].x, xb.y, acc.y);
-                        acc.y = fma(a[u].y, xb.x, acc.y);
-                    }
-                }
-                if (k < k1) {
-                    // the fewer than 8 entries left (all of a short row):
-                    // predicated, every load issued before the first is used
-                    uint64_t c[8];
-                    double2 a[8];
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) {
-                        c[u] = 0;
-                        a[u] = make_double2(0.0, 0.0);
-                        if (k + 8 * u < k1) {
-                            c[u] = __builtin_nontemporal_load(d.indices + k + 8 * u);
-                            const dvec2 av = __builtin_nontemporal_load(
-                                reinterpret_cast<const dvec2 *>(d.data + k + 8 * u));
-                            a[u] = make_double2(av.x, av.y);
-                        }
-                    }
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) {
-                        if (k + 8 * u < k1) {
-                            const double2 xb = d.x[c[u]];
-                            acc.x = fma(a[u].x, xb.x, acc.x);
-                            acc.x = fma(-a[u].y, xb.y, acc.x);
-                            acc.y = fma(a[u].x, xb.y, acc.y);
-                            acc.y = fma(a[u].y, xb.x, acc.y);
-                        }
-                    }
-                }
-            }
-#pragma unroll
-            for (int off = 4; off > 0; off /= 2) {
-                acc.x += __shfl_xor(acc.x, off, 8);
-                acc.y += __shfl_xor(acc.y, off, 8);
-            }
-            if (row < r1 && lane == 0) d.y[row] = acc;
-        }
+        if (d.G >= kMvNontemporalMinRow) batch_mv_rows<true>(d, r0, r1, lane);
+        else batch_mv_rows<false>(d, r0, r1, lane);
     }
 }
 

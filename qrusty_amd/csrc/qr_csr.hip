@@ -139,14 +139,17 @@ int launch_csr_build(const qr_op *op, const DeviceTables *t, uint64_t row_begin,
 // gathers coalesce.  Blocks are handed to the 8 XCDs round-robin; the block
 // index is remapped so that each XCD works on one contiguous range of rows
 // and its L2 sees the neighbouring rows' x lines.
-// CSR stream loads are nontemporal (each entry is read once)
-template <typename T>
+// CSR stream loads: nontemporal (kNT) for long rows, plain for short ones.
+// Measured (MI355X, complex128): nontemporal wins from about 256 entries per
+// row (400: 5%, 1000: 7%, 2000: 2.4%) and loses below (150: 3%, 60: 8%,
+// 25 -- the 24-qubit Heisenberg CSR: 13%).
+template <bool kNT, typename T>
 __device__ __forceinline__ T mv_load(const T *p) {
-    return __builtin_nontemporal_load(p);
+    if constexpr (kNT) return __builtin_nontemporal_load(p);
+    else return *p;
 }
 
-
-template <int LPR>
+template <int LPR, bool kNT>
 __global__ __launch_bounds__(256) void k_csr_matvec(uint64_t nrows, uint64_t ncols, uint64_t nnz,
                                                     uint32_t nblocks_xcd,
                                                     const uint64_t *__restrict__ indptr,
@@ -170,8 +173,8 @@ __global__ __launch_bounds__(256) void k_csr_matvec(uint64_t nrows, uint64_t nco
             double2 a[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                c[u] = mv_load(indices + k + u * LPR);
-                const dvec2 av = mv_load(reinterpret_cast<const dvec2 *>(data + k + u * LPR));
+                c[u] = mv_load<kNT>(indices + k + u * LPR);
+                const dvec2 av = mv_load<kNT>(reinterpret_cast<const dvec2 *>(data + k + u * LPR));
                 a[u] = make_double2(av.x, av.y);
             }
 #pragma unroll
@@ -194,8 +197,8 @@ __global__ __launch_bounds__(256) void k_csr_matvec(uint64_t nrows, uint64_t nco
                 c[u] = ~uint64_t(0);
                 a[u] = make_double2(0.0, 0.0);
                 if (k + u * LPR < k1) {
-                    c[u] = mv_load(indices + k + u * LPR);
-                    const dvec2 av = mv_load(reinterpret_cast<const dvec2 *>(data + k + u * LPR));
+                    c[u] = mv_load<kNT>(indices + k + u * LPR);
+                    const dvec2 av = mv_load<kNT>(reinterpret_cast<const dvec2 *>(data + k + u * LPR));
                     a[u] = make_double2(av.x, av.y);
                 }
             }
@@ -236,10 +239,15 @@ int launch_csr_matvec(uint64_t nrows, uint64_t ncols, uint64_t nnz, const uint64
     auto data = reinterpret_cast<const double2 *>(d_data);
     auto y = reinterpret_cast<double2 *>(d_y);
     const dim3 grid(static_cast<unsigned>(nblocks)), block(256);
+    const bool nt = avg >= kMvNontemporalMinRow;
 #define QR_MV_CASE(L)                                                                        \
     case L:                                                                                  \
-        hipLaunchKernelGGL(k_csr_matvec<L>, grid, block, 0, s, nrows, ncols, nnz, nbx,       \
-                           d_indptr, d_indices, data, x, y);                                 \
+        if (nt)                                                                              \
+            hipLaunchKernelGGL((k_csr_matvec<L, true>), grid, block, 0, s, nrows, ncols, nnz, \
+                               nbx, d_indptr, d_indices, data, x, y);                        \
+        else                                                                                 \
+            hipLaunchKernelGGL((k_csr_matvec<L, false>), grid, block, 0, s, nrows, ncols, nnz, \
+                               nbx, d_indptr, d_indices, data, x, y);                        \
         break;
     switch (lpr) {
         QR_MV_CASE(1)

@@ -39,6 +39,9 @@ int hip_fail(hipError_t e, const char *what);
 constexpr int kMaxDevices = 16;
 constexpr uint32_t kMaxHeavy = 4;  // heavy groups the CSR build folds per row
 constexpr uint32_t kSubMaxG = 2048;  // groups of one CSR-build sub-operator
+// CSR matvecs load the CSR stream nontemporally from this many entries per
+// row on (measured crossover; qr_csr.hip)
+constexpr uint64_t kMvNontemporalMinRow = 256;
 
 // Row-ordering entry: while building row r, a group whose sorted-x position is j
 // moves by `sigma` positions when bit `bit` of r is set (see build_rank_lists).
