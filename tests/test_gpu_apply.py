@@ -177,6 +177,7 @@ def test_apply_forms_vs_oracle(monkeypatch, kind, n, T, seed):
         assert np.linalg.norm(got - ref) <= 1e-12 * np.linalg.norm(ref)
         assert np.all(np.abs(got - ref) <= 1e-12 * _abs_bound(p, x, rows) + 1e-300)
         assert np.all(np.abs(e - eref) <= 1e-12 * np.abs(eref) + 1e-14)
+        assert np.array_equal(op.expectation(x, "fp64"), e)  # y never written
         ym, em = op.apply_and_expectation(x, "mixed")
         ry = np.linalg.norm(ym[:, rows.astype(np.int64)] - ref) / np.linalg.norm(ref)
         print(f"[gpu]   mixed rel err y {ry:.2e}", flush=True)
