@@ -100,8 +100,12 @@ int launch_csr_build(const qr_op *op, const DeviceTables *t, uint64_t row_begin,
         }
         return QR_OK;
     }
-    // G <= kSubMaxG: one or two group chunks per thread
-    const uint32_t jw = stage_jw_log2(G);
+    // G <= kSubMaxG: one or two group chunks per thread.  Operators of 129 to
+    // 512 groups run two chunks on half the lanes per row (twice the rows per
+    // batch): measured 7-13% faster at G = 150 and 400; one chunk stays
+    // faster at G = 25 and 60.
+    uint32_t jw = stage_jw_log2(G);
+    if (jw == 8 || jw == 9) jw -= 1;
     const uint32_t nch = (G + (1u << jw) - 1) >> jw;
     const uint32_t tl2 = 10 - jw + 4;
     const uint64_t nt = ((row_end - 1) >> tl2) - (row_begin >> tl2) + 1;

@@ -305,3 +305,19 @@ def test_sub_operators_with_heavy_group(n):
             cols, vals = orc.make_row(p, int(r))
             assert np.array_equal(got_ix, cols), f"row {r}"
             assert np.array_equal(got_dv.view(np.uint64), vals.view(np.uint64)), f"row {r}"
+
+
+@pytest.mark.parametrize("n,T,seed", [(15, 140, 1), (14, 300, 2), (15, 520, 3), (13, 64, 4)])
+def test_mid_size_ops_bit_exact(n, T, seed):
+    """Operators of 50..420 groups at 13..15 qubits (the single-operator build
+    runs 129..512 groups as two chunks on half the lanes per row)."""
+    rng = np.random.default_rng(2000 + seed)
+    labels = _random_labels(rng, n, T, prefixes=False)
+    coeffs = rng.standard_normal(T) + 1j * rng.standard_normal(T)
+    p = orc.Params(labels, coeffs)
+    print(f"[gpu] mid-size op n={n} T={T} G={len(set(p.x.tolist()))}", flush=True)
+    ip, ind, dat = orc.build_csr(p)
+    shape, data, indices, indptr = _export(_op(labels, coeffs).to_matrix_mode("Hip"))
+    assert indptr.tobytes() == ip.tobytes()
+    assert indices.tobytes() == ind.tobytes()
+    assert data.tobytes() == dat.tobytes()
