@@ -101,11 +101,12 @@ int launch_csr_build(const qr_op *op, const DeviceTables *t, uint64_t row_begin,
         return QR_OK;
     }
     // G <= kSubMaxG: one or two group chunks per thread.  Operators of 129 to
-    // 512 groups run two chunks on half the lanes per row (twice the rows per
-    // batch): measured 7-13% faster at G = 150 and 400; one chunk stays
-    // faster at G = 25 and 60.
+    // 1024 groups run two chunks on half the lanes per row (twice the rows per
+    // batch): measured 6-14% faster at G = 150, 417, 690, 880 and 990; one
+    // chunk stays faster at G = 25 and 60, and four chunks on 512 lanes were
+    // 22% slower than two on 1024 at G = 1996 (one block per CU instead of two).
     uint32_t jw = stage_jw_log2(G);
-    if (jw == 8 || jw == 9) jw -= 1;
+    if (jw == 8 || jw == 9 || (jw == 10 && G <= 1024)) jw -= 1;
     const uint32_t nch = (G + (1u << jw) - 1) >> jw;
     const uint32_t tl2 = 10 - jw + 4;
     const uint64_t nt = ((row_end - 1) >> tl2) - (row_begin >> tl2) + 1;

@@ -307,10 +307,11 @@ def test_sub_operators_with_heavy_group(n):
             assert np.array_equal(got_dv.view(np.uint64), vals.view(np.uint64)), f"row {r}"
 
 
-@pytest.mark.parametrize("n,T,seed", [(15, 140, 1), (14, 300, 2), (15, 520, 3), (13, 64, 4)])
+@pytest.mark.parametrize("n,T,seed", [(15, 140, 1), (14, 300, 2), (15, 520, 3), (13, 64, 4),
+                                      (14, 1100, 5)])
 def test_mid_size_ops_bit_exact(n, T, seed):
-    """Operators of 50..420 groups at 13..15 qubits (the single-operator build
-    runs 129..512 groups as two chunks on half the lanes per row)."""
+    """Operators of 50..850 groups at 13..15 qubits (the single-operator build
+    runs 129..1024 groups as two chunks on half the lanes per row)."""
     rng = np.random.default_rng(2000 + seed)
     labels = _random_labels(rng, n, T, prefixes=False)
     coeffs = rng.standard_normal(T) + 1j * rng.standard_normal(T)
