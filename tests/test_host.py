@@ -215,6 +215,10 @@ def test_spop_api():
     assert len(spop) == 2 and spop.num_qubits() == 4
     s2 = SparsePauliOp([Pauli("X")], [-0.5 - 1e-8j])
     assert repr(s2) == "SparsePauliOp('X', [-0.5+-0.00000001j])"
+    # `+` concatenates the term lists, a's terms first (qrusty/src/lib.rs:588-593)
+    s3 = SparsePauliOp([Pauli("IXXI")], [2.0 + 0.0j]) + spop
+    assert repr(s3) == "SparsePauliOp('IXXI','IIII','IXYZ', [2+0j, 1+0j, 1+0j])"
+    assert len(s3) == 3 and len(spop) == 2
 
 
 def test_slice_repr(hamiltonians):
