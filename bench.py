@@ -222,6 +222,10 @@ def bench_apply(args, torch, qa):
     E64 = e64.cpu().numpy().view(np.complex128)
     E32 = e32.cpu().numpy().view(np.complex128)
     rE = float(np.max(np.abs(E32 - E64) / np.abs(E64)))
+    # |dE| against the scale of the sum, ||psi|| ||H psi|| (the states are
+    # normalised): the error measure that cancellation in E does not inflate
+    nHpsi = torch.linalg.vector_norm(y64, dim=1).cpu().numpy()
+    rE_norm = float(np.max(np.abs(E32 - E64) / nHpsi))
     # the same FP64 path on the complex64-rounded states: separates the kernel's
     # arithmetic error from the rounding of psi itself (|E| of a traceless H in
     # a random state is small, so the latter dominates the relative error)
@@ -241,11 +245,13 @@ def bench_apply(args, torch, qa):
         "mixed_rel_err_y_vs_fp64": ry, "mixed_rel_err_expect_vs_fp64": rE,
         "mixed_rel_err_expect_vs_fp64_same_rounded_input": rE_kernel,
         "expect_rel_change_from_rounding_psi_to_c64": rE_input,
+        "mixed_err_expect_over_norm_Hpsi": rE_norm,
         "algorithmic_bytes": {"fp64": b64, "mixed": b32},
     }
     log(f"apply fp64 {ms64:.3f} ms ({res['fp64_GBps']:.0f} GB/s) mixed {ms32:.3f} ms "
         f"({res['mixed_GBps']:.0f} GB/s) speedup {ms64 / ms32:.2f}x err y {ry:.2e} E {rE:.2e} "
-        f"(kernel only {rE_kernel:.2e}, from rounding psi {rE_input:.2e})")
+        f"(kernel only {rE_kernel:.2e}, from rounding psi {rE_input:.2e}; "
+        f"|dE|/||H psi|| {rE_norm:.2e})")
     return res
 
 
