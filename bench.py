@@ -531,23 +531,37 @@ def bench_io(args, torch, qa):
 
 def bench_copy(args, torch):
     """Plain device-to-device copy of 2 GiB (torch copy_): the calibration
-    kernel whose known bytes scale the HBM counters FETCH_SIZE / WRITE_SIZE."""
+    kernel whose known bytes scale the HBM counters FETCH_SIZE / WRITE_SIZE.
+    Also the rates a write-only fill and a read-only reduction reach on 16 GiB,
+    the practical ceilings for the CSR build (writes) and matvec (reads)."""
+    def timed(f, steps):
+        for _ in range(args.warmup):
+            f()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(steps):
+            f()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / steps
+
     n = 1 << 28  # float64 elements = 2 GiB
     src = torch.ones(n, dtype=torch.float64, device="cuda")
     dst = torch.empty_like(src)
-    for _ in range(args.warmup):
-        dst.copy_(src)
-    torch.cuda.synchronize()
-    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    a.record()
-    for _ in range(args.steps):
-        dst.copy_(src)
-    b.record()
-    torch.cuda.synchronize()
-    ms = a.elapsed_time(b) / args.steps
+    ms = timed(lambda: dst.copy_(src), args.steps)
     res = {"workload": "device copy 2 GiB", "bytes_read": 8 * n, "bytes_written": 8 * n,
            "ms": ms, "GBps": 16 * n / ms / 1e6}
-    log(f"copy {ms:.3f} ms ({res['GBps']:.0f} GB/s)")
+    del src, dst
+    big = torch.empty(1 << 31, dtype=torch.float64, device="cuda")  # 16 GiB
+    ms_fill = timed(lambda: big.fill_(1.0), max(2, args.steps))
+    ms_sum = timed(lambda: big.sum(), max(2, args.steps))
+    res["fill_16GiB_GBps"] = 8 * big.numel() / ms_fill / 1e6
+    res["read_sum_16GiB_GBps"] = 8 * big.numel() / ms_sum / 1e6
+    del big
+    torch.cuda.empty_cache()
+    log(f"copy {ms:.3f} ms ({res['GBps']:.0f} GB/s); fill {res['fill_16GiB_GBps']:.0f} GB/s, "
+        f"read (sum) {res['read_sum_16GiB_GBps']:.0f} GB/s")
     return res
 
 
