@@ -27,6 +27,10 @@
 //    l = tid + 1024*k: the sign splits again into a per-thread part (bits of
 //    tid) and a per-k part (wave-uniform), so each pattern costs one FMA per
 //    row.
+//  * Partner-staged form (k_apply_staged): when many far groups share a
+//    partner tile (their masks agree above the tile), the groups are ordered
+//    by partner and each partner tile is loaded into LDS once, double-
+//    buffered, instead of being gathered once per group.
 //
 // Precision modes
 //   FP64 : complex128 state, FP64 FMAs (tile 2^12 rows, 4 rows per thread).
