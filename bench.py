@@ -621,6 +621,15 @@ def main():
         extra["vec"] = bench_vec(args, torch, qa)
     if args.only == "io":
         extra["io"] = bench_io(args, torch, qa)
+    if args.only is None and not args.no_extra:
+        # the rates this box sustains for plain streams, and the CSR kernels
+        # as a fraction of them (next to the fraction of the 8 TB/s peak)
+        cp = bench_copy(args, torch)
+        extra["copy"] = cp
+        if "csr" in extra:
+            extra["csr"]["build_frac_of_fill"] = extra["csr"]["build_GBps"] / cp["fill_16GiB_GBps"]
+            extra["csr"]["matvec_frac_of_read"] = extra["csr"]["matvec_GBps"] / cp["read_sum_16GiB_GBps"]
+            extra["csr"]["matvec_frac_of_copy"] = extra["csr"]["matvec_GBps"] / cp["GBps"]
     out["extra"] = extra
     print(json.dumps(out), flush=True)
 
