@@ -107,12 +107,27 @@ __host__ __device__ inline uint32_t heavy_fit(uint32_t nheavy, uint32_t jw_log2)
 }
 
 // Term-order fold of group j in row r: (-1)^popcount(r & z_t) c'_t, the
-// first value, then + (accel.rs:191-208).
+// first value, then + (accel.rs:191-208).  Terms are loaded four at a time,
+// all four loads issued before the first add; the adds stay in term order
+// (eight at a time cost the heavy-group instance its second block per CU).
+// Measured on the 24-qubit Heisenberg build: 2.68 -> 2.58 ms.
 template <bool kNarrow>
 __device__ __forceinline__ double2 fold_group(const TileOp &op, uint32_t j, uint64_t r) {
     const uint32_t t0 = op.gterm[j], t1 = op.gterm[j + 1];
     double2 v = signed_coeff(op.tc[t0], parity_and<kNarrow>(r, op.tz[t0]));
-    for (uint32_t t = t0 + 1; t < t1; ++t) {
+    uint32_t t = t0 + 1;
+    constexpr uint32_t kF = 4;
+    for (; t + kF <= t1; t += kF) {
+        double2 ct[kF];
+#pragma unroll
+        for (uint32_t u = 0; u < kF; ++u) ct[u] = signed_coeff(op.tc[t + u], parity_and<kNarrow>(r, op.tz[t + u]));
+#pragma unroll
+        for (uint32_t u = 0; u < kF; ++u) {
+            v.x = v.x + ct[u].x;
+            v.y = v.y + ct[u].y;
+        }
+    }
+    for (; t < t1; ++t) {
         const double2 ct = signed_coeff(op.tc[t], parity_and<kNarrow>(r, op.tz[t]));
         v.x = v.x + ct.x;
         v.y = v.y + ct.y;
