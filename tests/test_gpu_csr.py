@@ -255,6 +255,32 @@ def test_matvec_general_csr_shapes():
         _check_matvec(y, ref, abs(A).dot(np.abs(x)))
 
 
+def test_matvec_short_rows_ragged():
+    """16..32 entries per row on average (the two-rows-per-slot kernel) with
+    empty rows and rows of up to 300 entries, and a row count that is not a
+    multiple of the 64 rows a block covers."""
+    import scipy.sparse as sps
+    import qrusty_amd as qa
+    rng = np.random.default_rng(11)
+    for rows, cols in [(3001, 500), (64, 40), (130, 4000)]:
+        lens = rng.integers(0, 41, size=rows)
+        lens[rng.choice(rows, size=max(1, rows // 50), replace=False)] = rng.integers(33, 301)
+        lens[rng.choice(rows, size=max(1, rows // 20), replace=False)] = 0
+        lens = np.minimum(lens, cols)
+        indptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+        indices = np.concatenate([np.sort(rng.choice(cols, size=int(n), replace=False)) for n in lens]
+                                 + [np.zeros(0, dtype=np.int64)]).astype(np.int64)
+        data = rng.standard_normal(indptr[-1]) + 1j * rng.standard_normal(indptr[-1])
+        A = sps.csr_matrix((data, indices, indptr), shape=(rows, cols))
+        avg = indptr[-1] // rows
+        print(f"[gpu] short-row matvec rows={rows} cols={cols} avg={avg}", flush=True)
+        m = qa.SpMat.new_unchecked(A.shape, A.data, A.indices, A.indptr)
+        x = rng.standard_normal(cols) + 1j * rng.standard_normal(cols)
+        y = qa.spmat_dot_densevec(m, x)
+        ref = orc.spmat_dot_densevec(A.indptr.astype(np.uint64), A.indices.astype(np.uint64), A.data, x)
+        _check_matvec(y, ref, abs(A).dot(np.abs(x)))
+
+
 def test_device_tensor_matvec_stays_on_device(hamiltonians):
     import qrusty_amd as qa
     m = _op(*hamiltonians["py_H4"]).to_matrix_mode("Hip")
