@@ -101,7 +101,8 @@ __device__ __forceinline__ void batch_mv_rows(const BatchMvDesc &d, uint64_t r0,
         if (row < r1) {
             const uint64_t k1 = (row + 1) * d.G;
             uint64_t k = row * d.G + lane;
-            for (; k + 56 < k1; k += 64) {
+            // full chunks of 64 entries for all 8 lanes of the row together
+            for (; k - lane + 64 <= k1; k += 64) {
                 uint64_t c[8];
                 double2 a[8];
 #pragma unroll

@@ -171,9 +171,11 @@ __global__ __launch_bounds__(256) void k_csr_matvec(uint64_t nrows, uint64_t nco
     if (row < nrows) {
         const uint64_t k0 = min(indptr[row], nnz), k1 = min(indptr[row + 1], nnz);
         uint64_t k = k0 + lane;
-        // eight independent entries in flight per lane (measured best of 2, 4, 8)
+        // eight independent entries in flight per lane (measured best of 2, 4, 8);
+        // full chunks of U * LPR entries for all lanes of the row together, so
+        // that no lane of a row loops while another takes the tail
         constexpr int U = 8;
-        for (; k + (U - 1) * LPR < k1; k += U * LPR) {
+        for (; k - lane + U * LPR <= k1; k += U * LPR) {
             uint64_t c[U];
             double2 a[U];
 #pragma unroll
