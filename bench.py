@@ -215,6 +215,22 @@ def bench_apply(args, torch, qa):
     reps = max(3, args.steps)
     ms64 = timed(L.QR_PREC_FP64, x64, y64, e64, reps)
     ms32 = timed(L.QR_PREC_MIXED, x32, y32, e32, reps)
+
+    # expectation only: y is never written (algorithmic bytes nvec*N*s_in)
+    def timed_expect(prec, xt, et, reps):
+        L.check(L.lib.qr_pauli_apply(h.h, prec, nvec, P(xt), None, P(et), sh))
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        a.record(stream)
+        for _ in range(reps):
+            L.check(L.lib.qr_pauli_apply(h.h, prec, nvec, P(xt), None, P(et), sh))
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+    ee = torch.empty_like(e32)
+    ms32_e = timed_expect(L.QR_PREC_MIXED, x32, ee, reps)
+    ms64_e = timed_expect(L.QR_PREC_FP64, x64, ee, reps)
     b64 = nvec * N * 32
     b32 = nvec * N * 16
     ry = (torch.linalg.vector_norm((y32.to(torch.complex128) - y64).reshape(-1)) /
@@ -246,12 +262,14 @@ def bench_apply(args, torch, qa):
         "mixed_rel_err_expect_vs_fp64_same_rounded_input": rE_kernel,
         "expect_rel_change_from_rounding_psi_to_c64": rE_input,
         "mixed_err_expect_over_norm_Hpsi": rE_norm,
+        "expect_only_fp64_ms": ms64_e, "expect_only_mixed_ms": ms32_e,
+        "expect_only_mixed_pct_hbm": 100 * (b32 // 2) / ms32_e / 1e6 / HBM_PEAK_GBPS,
         "algorithmic_bytes": {"fp64": b64, "mixed": b32},
     }
     log(f"apply fp64 {ms64:.3f} ms ({res['fp64_GBps']:.0f} GB/s) mixed {ms32:.3f} ms "
         f"({res['mixed_GBps']:.0f} GB/s) speedup {ms64 / ms32:.2f}x err y {ry:.2e} E {rE:.2e} "
         f"(kernel only {rE_kernel:.2e}, from rounding psi {rE_input:.2e}; "
-        f"|dE|/||H psi|| {rE_norm:.2e})")
+        f"|dE|/||H psi|| {rE_norm:.2e}); expectation only: fp64 {ms64_e:.3f} ms, mixed {ms32_e:.3f} ms")
     return res
 
 
