@@ -7,9 +7,9 @@
 // entries: D_g(r) is the CSR value of group g in row r, x[r ^ x_g] the gather
 // of its column.
 //
-// Work decomposition (one block of 512 threads = one tile of 2^L consecutive
-// rows of one state vector; two blocks share a CU, so that one computes while
-// the other waits at a barrier or on its loads):
+// Work decomposition (one block of 256 threads = one tile of 2^L consecutive
+// rows of one state vector; four blocks share a CU, so that some compute
+// while others wait at a barrier or on their loads):
 //  * The tile's slice of x is staged in LDS (coalesced); groups whose x mask
 //    is below 2^L gather from LDS, the others ("far" groups) gather the
 //    partner tile t ^ (x_g >> L) from global memory (coalesced, permuted
@@ -25,7 +25,7 @@
 //    uniform), and per row only patterns with zl != 0 cost a sign.  The block
 //    forms all its patterns' coefficients in parallel at the start of the
 //    tile and keeps them in LDS next to the x tile.  Thread `tid` owns rows
-//    l = tid + 512*k: the sign splits again into a per-thread part (bits of
+//    l = tid + 256*k: the sign splits again into a per-thread part (bits of
 //    tid) and a per-k part (wave-uniform), so each pattern costs one FMA per
 //    row.
 //  * Partner-staged form (k_apply_staged): when many far groups share a
@@ -34,8 +34,8 @@
 //    buffered, instead of being gathered once per group.
 //
 // Precision modes
-//   FP64 : complex128 state, FP64 FMAs (tile 2^11 rows, 4 rows per thread).
-//   MIXED: complex64 state (tile 2^12 rows, 8 rows per thread).  D_g(r) is
+//   FP64 : complex128 state, FP64 FMAs (tile 2^10 rows, 4 rows per thread).
+//   MIXED: complex64 state (tile 2^11 rows, 8 rows per thread).  D_g(r) is
 //          merged in FP64 -- where terms sharing an x mask cancel -- and then
 //          rounded to FP32.  Products and the sum over groups run in FP32:
 //          products are accumulated into a block partial over up to
@@ -51,21 +51,24 @@
 
 namespace qr {
 
-// 512 threads and at most 80 KB of LDS per block: two blocks per CU (the
-// 160 KB of a gfx950 CU).  Measured against one 1024-thread block per CU
-// with twice the tile (24-qubit Heisenberg, 8 states: mixed 1.80 -> 1.75 ms,
-// FP64 3.75 -> 3.66 ms; 28-qubit k-local: 252 -> 237 ms, 422 -> 398 ms).
-constexpr int kApplyThreads = 512;
-constexpr int kApplyThreadBits = 9;
+// 256 threads and at most 40 KB of LDS per block: four blocks per CU (the
+// 160 KB of a gfx950 CU).  Measured (24-qubit Heisenberg, 8 states; 28-qubit
+// k-local, one state), by threads per block and tile:
+//   1024, 2^13 / 2^12 rows (mixed / FP64): 1.80 / 3.75 ms; 252 / 422 ms
+//    512, 2^12 / 2^11 rows:                1.75 / 3.66 ms; 237 / 398 ms
+//    256, 2^11 / 2^10 rows:                1.64 / 3.47 ms; 228 / 400 ms
+// although each halving of the tile adds a far group.
+constexpr int kApplyThreads = 256;
+constexpr int kApplyThreadBits = 8;
 constexpr int kCompBlock = 16;  // groups per FP32 partial between TwoSum steps
-constexpr size_t kApplyMaxLDS = 80 * 1024;  // LDS of one block
+constexpr size_t kApplyMaxLDS = 40 * 1024;  // LDS of one block
 
-// Tile of 2^L rows (x tile in LDS: 32 KB in both precisions), R rows per
+// Tile of 2^L rows (x tile in LDS: 16 KB in both precisions), R rows per
 // thread, and the LDS left for one chunk of pattern coefficients (20 B each)
 // and the row-bit table (R x 16 B per row, plus a zero row).
 template <bool kMixed>
 struct ApplyShape {
-    static constexpr int L = kMixed ? 12 : 11;
+    static constexpr int L = kMixed ? 11 : 10;
     static constexpr int R = (1 << L) / kApplyThreads;
     static constexpr size_t kTile = (size_t(1) << L) * (kMixed ? 8 : 16);
     // LDS after the x tile: 5/8 for patterns, 3/8 for the row-bit table
@@ -529,7 +532,7 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_tiled(
 // the tile itself), so a partner tile is read once per chunk however many
 // groups use it.  The load of run r + 1 (LDS-DMA: global memory straight into
 // LDS, no registers) is in flight while run r is computed.
-constexpr uint32_t kStageTileBytes = 32768;  // one tile: 2^12 complex64 or 2^11 complex128
+constexpr uint32_t kStageTileBytes = 16384;  // one tile: 2^11 complex64 or 2^10 complex128
 
 // One LDS-DMA load: each lane reads the 16 bytes at its `src`, and the wave
 // writes them to LDS at lds + lane * 16 (lds: LDS byte address, wave-uniform,
@@ -781,22 +784,20 @@ static int launch_staged(const qr_op *op, DeviceTables *t, uint32_t nvec, const 
 // tile once per run instead of gathering it once per group, but the load of a
 // one-group run is not hidden behind much work, and while the state is small
 // enough to stay in the caches the per-group gathers are cheap.  Measured
-// with 512-thread blocks (F far groups on D distinct partner offsets; time
-// of the staged form against the gather form):
-//   mixed (2^12-row tiles):
-//     24-qubit k-local, 100..3000 terms: F/D 1.9..4.9, 1.17..1.41x faster;
-//     28-qubit k-local, 10000 terms: 1.33x faster;
-//     Heisenberg ring, 22 and 24 qubits: F/D 1, 0.95..0.99x;
-//     20-qubit random (8 MB state): F/D 1.1..1.5 -> 0.83..1.07x,
-//     4.0 -> 1.13x, 7.8 -> 1.19x;
-//   FP64 (2^11-row tiles):
-//     24-qubit k-local: F/D 1.8..4.1, 1.28..1.90x faster; 28-qubit: 1.33x;
-//     Heisenberg ring, 22 and 24 qubits: F/D 1, 1.04..1.05x faster;
-//     20-qubit random (16 MB state): F/D 1.05..2.3 -> 1.30..1.47x,
-//     4.0 -> 0.93x.
-// Hence, for states of at least 32 MiB per vector: mixed staged when
-// F >= 1.5 D, FP64 always; below that, mixed when F >= 4 D, FP64 when
-// F < 3.5 D.  The environment variable QR_APPLY_FORM (1 gather, 2 staged)
+// with the tiles above (F far groups on D distinct partner offsets; time of
+// the staged form against the gather form):
+//   mixed (2^11-row tiles):
+//     24-qubit k-local, 100..3000 terms: F/D 1.8..4.1, 1.24..1.46x faster;
+//     28-qubit k-local, 10000 terms: 1.40x faster;
+//     Heisenberg ring, 22 and 24 qubits: F/D 1, 0.96..1.01x;
+//     20-qubit random (8 MB state): F/D 1.05..1.24 -> 0.80..1.00x,
+//     2.3 -> 1.06x, 4.0 -> 1.12x;
+//   FP64 (2^10-row tiles): faster in every case measured -- k-local
+//     1.29..1.86x, 28-qubit k-local 1.40x, Heisenberg ring 1.07..1.10x,
+//     20-qubit random (F/D 1.0..2.3) 1.11..1.36x.
+// Hence: FP64 staged whenever a group reaches outside the tile; mixed staged
+// when F >= 1.5 D for states of at least 32 MiB per vector, F >= 2 D below
+// that.  The environment variable QR_APPLY_FORM (1 gather, 2 staged)
 // overrides the choice, for tests and measurement.
 static bool prefer_staged(const qr_op *op, int L, size_t elem_bytes) {
     const char *e = getenv("QR_APPLY_FORM");
@@ -812,9 +813,9 @@ static bool prefer_staged(const qr_op *op, int L, size_t elem_bytes) {
         prev = hi;
     }
     if (far == 0) return false;
+    if (elem_bytes == 16) return true;
     const bool big = (uint64_t(1) << op->nq) * elem_bytes >= (uint64_t(32) << 20);
-    if (elem_bytes == 8) return big ? 2 * far >= 3 * distinct : far >= 4 * distinct;
-    return big || 2 * far < 7 * distinct;
+    return big ? 2 * far >= 3 * distinct : far >= 2 * distinct;
 }
 
 int launch_pauli_apply(const qr_op *op, DeviceTables *t, int32_t precision, uint32_t nvec,
