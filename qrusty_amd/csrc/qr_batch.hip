@@ -84,7 +84,10 @@ struct BatchMvDesc {
     double2 *y;
     uint64_t rows, G, item_begin;
 };
-constexpr uint32_t kMvRows = 1024;
+// 128 rows per item: the 256-operator uneven batch measured 100 ms per step
+// at 1024 rows, 94-95 at 512, 93.6 at 256, 92.5-94 at 128, 93 at 64 and 32
+// (smaller items leave less of a tail).
+constexpr uint32_t kMvRows = 128;
 
 // Rows [r0, r1) of one operator's y = A x, 8 lanes per row; the CSR stream is
 // loaded nontemporally (kNT) for long rows only, as in k_csr_matvec.
