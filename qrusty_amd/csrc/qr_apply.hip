@@ -820,7 +820,11 @@ static bool prefer_staged(const qr_op *op, int L, size_t elem_bytes) {
 
 int launch_pauli_apply(const qr_op *op, DeviceTables *t, int32_t precision, uint32_t nvec,
                        const void *d_x, void *d_y, double *d_expect, hipStream_t s) {
-    const bool mixed = precision == QR_PREC_MIXED, narrow = op->nq <= 32;
+    // QR_APPLY_WIDE=1 runs the 64-bit-mask instances on any operator, so that
+    // the tests can check them without a state of more than 32 qubits
+    const char *wide_env = getenv("QR_APPLY_WIDE");
+    const bool force_wide = wide_env && atoi(wide_env) == 1;
+    const bool mixed = precision == QR_PREC_MIXED, narrow = op->nq <= 32 && !force_wide;
     const int Lmax = mixed ? ApplyShape<true>::L : ApplyShape<false>::L;
     const int L = int(op->nq) < Lmax ? int(op->nq) : Lmax;
     const bool staged = L == Lmax && prefer_staged(op, L, mixed ? 8 : 16);

@@ -182,3 +182,26 @@ def test_apply_forms_vs_oracle(monkeypatch, kind, n, T, seed):
         ry = np.linalg.norm(ym[:, rows.astype(np.int64)] - ref) / np.linalg.norm(ref)
         print(f"[gpu]   mixed rel err y {ry:.2e}", flush=True)
         assert ry <= 1e-6
+
+
+@pytest.mark.parametrize("form", ["1", "2"])
+def test_wide_mask_instances(monkeypatch, form):
+    """The 64-bit-mask kernel instances (used from 33 qubits on) on a 14-qubit
+    operator, forced with QR_APPLY_WIDE, in both forms, against the oracle."""
+    from qrusty_amd import workloads
+    labels, coeffs = workloads.k_local_op(14, 300, seed=81)
+    p = orc.Params(labels, coeffs)
+    x = _states(p.N, 2, seed=82)
+    monkeypatch.setenv("QR_APPLY_WIDE", "1")
+    monkeypatch.setenv("QR_APPLY_FORM", form)
+    print(f"[gpu] apply wide masks form={form}", flush=True)
+    op = _op(labels, coeffs)
+    y, e = op.apply_and_expectation(x, "fp64")
+    rows = np.arange(p.N, dtype=np.uint64)
+    ref = orc.apply_rows(p, x, rows)
+    assert np.linalg.norm(y - ref) <= 1e-12 * np.linalg.norm(ref)
+    assert np.all(np.abs(y - ref) <= 1e-12 * _abs_bound(p, x, rows) + 1e-300)
+    eref = orc.vdot(x, orc.apply_full(p, x))
+    assert np.all(np.abs(e - eref) <= 1e-12 * np.abs(eref) + 1e-14)
+    ym = op.apply(x, "mixed")
+    assert np.linalg.norm(ym - ref) <= 1e-6 * np.linalg.norm(ref)
