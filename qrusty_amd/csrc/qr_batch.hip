@@ -38,9 +38,12 @@ struct BatchOpDesc {
 // items of that class only, taken from the class's own counter.
 // Minimum waves per SIMD requested from the compiler: 8 means two resident
 // 1024-thread blocks per CU (at most 64 VGPRs), as the single-operator build
-// runs; 1 leaves the register budget alone.
+// runs; 1 leaves the register budget alone.  Only the one-chunk build fits
+// 64 VGPRs here: the two-chunk build spilled 25 registers there (its
+// item loop costs a few more than the single-operator kernel), and without
+// the bound the 256-operator batch measured 92.0-93.1 against 93.0-94.1 ms.
 #ifndef QR_BATCH_WAVES
-#define QR_BATCH_WAVES(matvec, chunks) ((!(matvec) && (chunks) <= 2) ? 8 : 1)
+#define QR_BATCH_WAVES(matvec, chunks) ((!(matvec) && (chunks) <= 1) ? 8 : 1)
 #endif
 template <bool kNarrow, int kChunks, bool kMatvec>
 __global__ __launch_bounds__(kStageThreads, QR_BATCH_WAVES(kMatvec, kChunks)) void k_batch(const BatchOpDesc *__restrict__ ops,
