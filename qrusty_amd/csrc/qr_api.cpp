@@ -119,9 +119,10 @@ static void free_tables(DeviceTables &t) {
 // thread-index bits and the row-index bits above them; a pattern whose zl
 // lies in the thread bits costs once per thread, one whose zl lies in the
 // row bits once per block (a small per-tile table), and only the straddling
-// ones cost per row.  Terms keep input order inside a pattern.
+// ones cost per row.  Terms keep input order inside a pattern.  With `pairs`
+// the low bits are taken in the paired-row order (pair_row_bits).
 void build_patterns(const qr_op *op, int L, int tb, uint32_t pcap, uint32_t ucap, bool staged,
-                    HostPatterns &out) {
+                    bool pairs, HostPatterns &out) {
     const uint64_t mask = L >= 64 ? ~uint64_t(0) : ((uint64_t(1) << L) - 1);
     const uint64_t tmask = (uint64_t(1) << tb) - 1;
     out = HostPatterns{};
@@ -195,7 +196,7 @@ void build_patterns(const qr_op *op, int L, int tb, uint32_t pcap, uint32_t ucap
         std::vector<uint64_t> zls;
         std::vector<std::vector<uint32_t>> members;
         for (uint32_t i = op->gterm[g]; i < op->gterm[g + 1]; ++i) {
-            const uint64_t zl = op->tz[i] & mask;
+            const uint64_t zl = pairs ? pair_row_bits(op->tz[i] & mask, tb) : op->tz[i] & mask;
             size_t k = 0;
             while (k < zls.size() && zls[k] != zl) ++k;
             if (k == zls.size()) { zls.push_back(zl); members.emplace_back(); }
@@ -276,7 +277,7 @@ int device_tables(const qr_op *op, DeviceTables **out) {
 }
 
 int device_patterns(const qr_op *op, DeviceTables *t, int slot, int L, int tb, uint32_t pcap,
-                    uint32_t ucap, bool staged) {
+                    uint32_t ucap, bool staged, bool pairs) {
     std::lock_guard<std::mutex> lock(op->mu);
     DeviceTables::Patterns &pt = t->pat[slot];
     if (pt.L == L) return QR_OK;
@@ -285,7 +286,7 @@ int device_patterns(const qr_op *op, DeviceTables *t, int slot, int L, int tb, u
         if (p) (void)hipFree(p);
     pt = DeviceTables::Patterns{};
     HostPatterns hp;
-    build_patterns(op, L, tb, pcap, ucap, staged, hp);
+    build_patterns(op, L, tb, pcap, ucap, staged, pairs, hp);
     std::vector<double2> c2(hp.ptz.size());
     for (size_t i = 0; i < c2.size(); ++i) c2[i] = make_double2(hp.ptc[2 * i], hp.ptc[2 * i + 1]);
     int rc;

@@ -27,7 +27,10 @@
 //    tile and keeps them in LDS next to the x tile.  Thread `tid` owns rows
 //    l = tid + 256*k: the sign splits again into a per-thread part (bits of
 //    tid) and a per-k part (wave-uniform), so each pattern costs one FMA per
-//    row.
+//    row.  The gather form on full complex64 tiles owns row pairs instead,
+//    l = 2*tid + (k & 1) + 512*(k >> 1), so that one 16-byte load gathers
+//    both partners of a pair and one 16-byte store writes both results; the
+//    patterns' low z bits are permuted to match (row bit 0 counts with k).
 //  * Partner-staged form (k_apply_staged): when many far groups share a
 //    partner tile (their masks agree above the tile), the groups are ordered
 //    by partner and each partner tile is loaded into LDS once, double-
@@ -114,10 +117,43 @@ constexpr int kBufferWord3 = 0x00020000;
 // Gathers of x[r ^ m] for the R rows of a thread: from the LDS x tile, or --
 // for masks above the tile -- from the partner tile in global memory, with
 // one uniform base pointer per row block and the thread's offset shared.
-template <bool kFar, bool kFull, int R, typename cT>
+// With kPairs (complex64, full tiles) the thread's rows k = 2j, 2j+1 are the
+// adjacent rows (2 tid, 2 tid + 1) + 2^9 j: their partners r ^ m form one
+// aligned pair, read with one 16-byte load and swapped when bit 0 of m is set.
+template <bool kFar, bool kFull, bool kPairs, int R, typename cT>
 __device__ __forceinline__ void gather(uint64_t m, uint32_t tid, uint64_t tile_rows, uint64_t base,
                                        const cT *xt, const cT *__restrict__ xv, cT (&xg)[R]) {
     const uint64_t mlo = m & (tile_rows - 1), mhi = m & ~(tile_rows - 1);
+    if constexpr (kPairs) {
+        static_assert(sizeof(cT) == 8 && kFull && R % 2 == 0, "pairs: complex64, full tiles");
+        constexpr int kJ = kApplyThreadBits + 1;  // row bit of the pair index j
+        const uint32_t tq = tid ^ uint32_t((mlo >> 1) & (kApplyThreads - 1));
+        const uint32_t jm = uint32_t(mlo >> kJ);
+        const bool swap = (mlo & 1) != 0;
+        float4 pr[R / 2];
+        if (kFar) {
+            const cT *far = xv + (base ^ mhi);
+            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                const_cast<cT *>(far), 0, int(tile_rows * sizeof(cT)), kBufferWord3);
+            const uint32_t voff = tq * uint32_t(2 * sizeof(cT));
+#pragma unroll
+            for (int j = 0; j < R / 2; ++j) {
+                const uint32_t soff = ((uint32_t(j) ^ jm) << kJ) * uint32_t(sizeof(cT));
+                pr[j] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, 0));
+            }
+        } else {
+            const float4 *xrow = reinterpret_cast<const float4 *>(xt) + tq;
+#pragma unroll
+            for (int j = 0; j < R / 2; ++j) pr[j] = xrow[(uint32_t(j) ^ jm) << kApplyThreadBits];
+        }
+#pragma unroll
+        for (int j = 0; j < R / 2; ++j) {
+            const float2 lo = make_float2(pr[j].x, pr[j].y), hi = make_float2(pr[j].z, pr[j].w);
+            xg[2 * j] = swap ? hi : lo;
+            xg[2 * j + 1] = swap ? lo : hi;
+        }
+        return;
+    }
     const uint32_t tx = tid ^ uint32_t(mlo & (kApplyThreads - 1));
     const uint32_t mk = uint32_t(mlo >> kApplyThreadBits);
     if (kFar) {
@@ -401,6 +437,9 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_tiled(
     double *__restrict__ partials) {
     using Shape = ApplyShape<kMixed>;
     constexpr int L = Shape::L, R = Shape::R;
+    // complex64 full tiles: paired rows (see gather); the patterns are built
+    // in the paired-row bit order to match
+    constexpr bool kPairs = kMixed && kFull;
     using cT = typename std::conditional<kMixed, float2, double2>::type;
     extern __shared__ __align__(16) unsigned char smem_raw[];
 
@@ -463,7 +502,7 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_tiled(
                     // so that no later scalar-load wait also waits for them
                     asm volatile("" ::"s"(gd.p0), "s"(gd.ps), "s"(gd.pk), "s"(gd.p1), "s"(gd.u),
                                  "s"(gd.flags));
-                    gather<kFar, kFull, R>(gd.m, tid, tile_rows, base, xt, xv, xg);
+                    gather<kFar, kFull, kPairs, R>(gd.m, tid, tile_rows, base, xt, xv, xg);
                     accumulate<kMixed, kFull, kRowvar, kCplx, R>(gd, tid, tile_rows, zero_row, sA, sZ,
                                                                  sU, xg, ar, ai, br, bi);
                 }
@@ -493,7 +532,9 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_tiled(
     cT *__restrict__ yv = yout ? reinterpret_cast<cT *>(yout) + uint64_t(v) * N : nullptr;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-        const uint64_t l = tid + uint64_t(k) * kApplyThreads;
+        const uint64_t l = kPairs ? ((uint64_t(tid) << 1) | uint64_t(k & 1) |
+                                     (uint64_t(k >> 1) << (kApplyThreadBits + 1)))
+                                  : tid + uint64_t(k) * kApplyThreads;
         if (!kFull && l >= tile_rows) break;
         double yr, yi;
         if (!kMixed) {
@@ -503,7 +544,21 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_tiled(
         } else {
             yr = double(sr[k]) + double(cr[k]);
             yi = double(si[k]) + double(ci[k]);
-            if (yv) store_y(reinterpret_cast<float2 *>(yv) + base + l, make_float2(sr[k] + cr[k], si[k] + ci[k]));
+            if (yv) {
+                if constexpr (kPairs) {
+                    // the pair's two rows in one 16-byte store, at the odd row
+                    if (k & 1) {
+                        typedef float f4v __attribute__((ext_vector_type(4)));
+                        const f4v pv = {sr[k - 1] + cr[k - 1], si[k - 1] + ci[k - 1],
+                                        sr[k] + cr[k], si[k] + ci[k]};
+                        __builtin_nontemporal_store(
+                            pv, reinterpret_cast<f4v *>(reinterpret_cast<float2 *>(yv) + base + l - 1));
+                    }
+                } else {
+                    store_y(reinterpret_cast<float2 *>(yv) + base + l,
+                            make_float2(sr[k] + cr[k], si[k] + ci[k]));
+                }
+            }
         }
         if (partials) {
             const cT xr = xt[l];
@@ -627,7 +682,7 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_staged(
                     const ApplyGroup gd = grp[g];
                     asm volatile("" ::"s"(gd.p0), "s"(gd.ps), "s"(gd.pk), "s"(gd.p1), "s"(gd.u),
                                  "s"(gd.flags));
-                    gather<false, true, R>(gd.m, tid, tile_rows, base, xt, xv, xg);
+                    gather<false, true, false, R>(gd.m, tid, tile_rows, base, xt, xv, xg);
                     accumulate<kMixed, true, kRowvar, kCplx, R>(gd, tid, tile_rows, zero_row, sA, sZ,
                                                                 sU, xg, ar, ai, br, bi);
                 }
@@ -832,8 +887,10 @@ int launch_pauli_apply(const qr_op *op, DeviceTables *t, int32_t precision, uint
                                  : (mixed ? ApplyShape<true>::kPatChunk : ApplyShape<false>::kPatChunk);
     const uint32_t ucap = staged ? (mixed ? ApplyShape<true>::kTableRowsS : ApplyShape<false>::kTableRowsS)
                                  : (mixed ? ApplyShape<true>::kTableRows : ApplyShape<false>::kTableRows);
+    // the gather form on full complex64 tiles runs the paired-row layout
+    const bool pairs = mixed && !staged && L == Lmax;
     int rc = device_patterns(op, t, (mixed ? 1 : 0) + (staged ? 2 : 0), L, kApplyThreadBits, pcap,
-                             ucap, staged);
+                             ucap, staged, pairs);
     if (rc) return rc;
     const uint64_t N = uint64_t(1) << op->nq;
     if ((N >> L) * uint64_t(nvec) > 0x7FFFFFFFull || nvec == 0)

@@ -135,10 +135,18 @@ struct HostPatterns {
     std::vector<ApplyRun> run;  // staged form only
     uint32_t max_p = 0, max_u = 0, nfar = 0;
 };
+// Paired-row layout of the mixed gather form: thread t owns the row pairs
+// (2t, 2t+1) + 2^(tb+1) j, so row-bit 0 counts with the row index (bit 0 of
+// k) and row-bits 1..tb are the thread bits.  The patterns then hold z in
+// that order: bits 1..tb moved down to 0..tb-1, bit 0 up to bit tb.
+inline uint64_t pair_row_bits(uint64_t zl, int tb) {
+    const uint64_t tmask = (uint64_t(1) << tb) - 1;
+    return ((zl >> 1) & tmask) | ((zl & 1) << tb) | (zl & ~((tmask << 1) | 1));
+}
 void build_patterns(const qr_op *op, int L, int tb, uint32_t pcap, uint32_t ucap, bool staged,
-                    HostPatterns &out);
+                    bool pairs, HostPatterns &out);
 int device_patterns(const qr_op *op, DeviceTables *t, int slot, int L, int tb, uint32_t pcap,
-                    uint32_t ucap, bool staged);
+                    uint32_t ucap, bool staged, bool pairs);
 
 }  // namespace qr
 
