@@ -49,12 +49,27 @@ def load_traffic(kernel: str, workload: str):
         return None
 
 
-def cpu_threads() -> int:
+def cpu_info() -> dict:
+    """The host CPUs this process may use: every core of its affinity mask,
+    capped at the CPU share the machine grants one GPU job when the
+    environment states one (OMP_NUM_THREADS), plus nproc and the CPU model."""
     try:
-        n = len(os.sched_getaffinity(0))
+        affinity = len(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(16, n))
+        affinity = os.cpu_count() or 1
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(affinity, int(share)) if share.isdigit() and int(share) > 0 else affinity
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"threads": max(1, threads), "affinity_cpus": affinity, "nproc": os.cpu_count(),
+            "omp_num_threads": share or None, "cpu_model": model}
 
 
 # ---------------------------------------------------------------------------
@@ -149,11 +164,11 @@ def bench_csr(args, torch, qa):
 def cpu_baseline_csr(labels, coeffs, target_s=4.0):
     """Oracle (C restatement of RowwiseUnsafeChunked/1000 + spmat_dot_densevec)
     on a bounded sample of rows; median of 3 runs after 1 warm-up."""
-    import numpy as np
     from oracle import oracle as orc
     from qrusty_amd import workloads
     p = orc.Params(labels, coeffs)
-    nthr = cpu_threads()
+    info = cpu_info()
+    nthr = info["threads"]
     x = workloads.random_states(p.N, 1, seed=workloads.SEED_CFG2 + 1)[0]
 
     def run(rows):
@@ -173,8 +188,11 @@ def cpu_baseline_csr(labels, coeffs, target_s=4.0):
         times.append(t)
     med = statistics.median(times)
     return {"value": nnz / med, "unit": "nnz/s", "cores": nthr, "kind": "port",
-            "sample": f"rows 0..{rows} of 2^20 ({nnz} nnz): RowwiseUnsafeChunked/1000 build "
-                      f"+ spmat_dot_densevec, median of 3 after 1 warm-up, {nthr} threads"}
+            "sample": f"rows 0..{rows} of 2^20 ({nnz} nnz, {100 * rows / p.N:.1f}% of the rows): "
+                      f"RowwiseUnsafeChunked/1000 build + spmat_dot_densevec, median of 3 after "
+                      f"1 warm-up, {nthr} threads",
+            "full_row_set": rows == p.N, "rows_timed": rows, "rows_total": p.N,
+            "median_s": med, "host": info}
 
 
 # ---------------------------------------------------------------------------
@@ -303,17 +321,24 @@ def bench_batch(args, torch, qa):
     torch.cuda.synchronize()
     ms = [ev[k].elapsed_time(ev[k + 1]) for k in range(args.steps)]
     m = statistics.mean(ms)
-    # algorithmic bytes: CSR written (24 B/nnz + indptr), then y = A x (CSR
-    # re-read by the single-operator route only; the fused route reads x)
-    big = [k for k, h in enumerate(b._h) if h.G > 6144]
+    # algorithmic bytes: CSR written (24 B/nnz + indptr), then y = A x (x read,
+    # y written).  The split form's matvec pass reads every CSR back; the
+    # fused form reads back only the operators of more than 6144 groups, which
+    # run the single-operator kernels.  In the split form every operator of
+    # more than 2048 groups is built by the single-operator (sub-operator)
+    # kernels.
+    split = bool(best.get("split", 0))
+    big = [k for k, h in enumerate(b._h) if h.G > (2048 if split else 6144)]
     nnz_big = sum(b._h[k].nnz for k in big)
-    byts = 24 * nnz + 8 * (rows + len(ops)) + 24 * nnz_big + 32 * rows
+    nnz_reread = nnz if split else sum(h.nnz for h in b._h if h.G > 6144)
+    byts = 24 * nnz + 8 * (rows + len(ops)) + 24 * nnz_reread + 32 * rows
     res = {"workload": "batch of 256 uneven random operators (8..18 qubits, 10..5000 terms): "
                        "CSR build + one matvec each, one graph launch",
            "operators": len(ops), "rows": rows, "nnz": nnz, "ms_per_step": m,
            "nnz_per_s": nnz / (m / 1e3), "GBps": byts / m / 1e6,
            "pct_hbm": 100 * byts / m / 1e6 / HBM_PEAK_GBPS, "tuning": best, "tune_s": tune_s,
-           "single_operator_route": {"operators": len(big), "nnz_frac": nnz_big / nnz}}
+           "single_operator_route": {"operators": len(big), "nnz_frac": nnz_big / nnz},
+           "csr_read_back_nnz": nnz_reread}
     if not args.no_check:
         # invariant check: every operator's indptr is r*G and one sampled row per
         # operator holds sorted r ^ x_g

@@ -277,7 +277,11 @@ int qr_mm_read_close(qr_mm_reader *reader);
  * d_expect (optional, may be NULL): 2*nvec doubles (device) receiving
  * <x_v | y_v> = sum_r conj(x_v[r]) y_v[r] in FP64.
  * d_y may be NULL when d_expect is given: then only <x|H|x> is produced and
- * y is never written.
+ * y is never written.  y must not overlap x (QR_ERR_ARG): the apply is not in
+ * place, since blocks gather x rows that other blocks' y rows would replace.
+ * Each launch that computes <x|H|x> takes its own workspace for the block
+ * partials from a per-device stream-ordered pool, so calls on one qr_op from
+ * several threads and streams may overlap.
  * Two kernel forms give the same results within rounding (the order of the
  * sum over groups differs): one gathers partner tiles per group, the other
  * loads each partner tile into LDS once for all the groups that read it.

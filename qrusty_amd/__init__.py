@@ -291,8 +291,13 @@ class SparsePauliOp:
         y = None
         if want_y:
             y = out if out is not None else torch.empty_like(xt)
+            if not isinstance(y, torch.Tensor) or not y.is_cuda or y.device != xt.device:
+                raise QrError("apply: out must be a device tensor on the state's device")
             if y.shape != xt.shape or y.dtype != dt or not y.is_contiguous():
                 raise QrError("apply: out has the wrong shape, dtype or layout")
+            nb = xt.numel() * xt.element_size()
+            if y.data_ptr() < xt.data_ptr() + nb and xt.data_ptr() < y.data_ptr() + nb:
+                raise QrError("apply: out overlaps the state (the apply is not in place)")
         e = torch.empty(2 * nvec, dtype=torch.float64, device=xt.device) if want_e else None
         L.check(L.lib.qr_pauli_apply(h.h, prec, nvec, _ptr(xt),
                                      _ptr(y) if y is not None else None,

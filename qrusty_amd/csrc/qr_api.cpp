@@ -100,8 +100,7 @@ static void build_rank_lists(const std::vector<uint64_t> &gx, std::vector<uint32
 }
 
 static void free_tables(DeviceTables &t) {
-    void *ptrs[] = {t.gx, t.gterm, t.tz, t.tc, t.rl_ptr, t.rl, t.gkind, t.heavy, t.leaf_heavy,
-                    t.partials};
+    void *ptrs[] = {t.gx, t.gterm, t.tz, t.tc, t.rl_ptr, t.rl, t.gkind, t.heavy, t.leaf_heavy};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &pt : t.pat) {
@@ -759,6 +758,13 @@ int qr_pauli_apply(const qr_op *op, int32_t precision, uint32_t nvec, const void
         return fail(QR_ERR_ARG, "unknown precision");
     if (nvec == 0) return QR_OK;
     if (op->nq > 36) return fail(QR_ERR_SIZE, "states of more than 36 qubits are not supported");
+    if (d_y) {
+        // y is written while other blocks still gather x: no overlap allowed
+        const uint64_t bytes = uint64_t(nvec) << (op->nq + (precision == QR_PREC_MIXED ? 3 : 4));
+        const uintptr_t x0 = uintptr_t(d_x), y0 = uintptr_t(d_y);
+        if (x0 < y0 + bytes && y0 < x0 + bytes)
+            return fail(QR_ERR_ARG, "apply: y overlaps x (the apply is not in place)");
+    }
     DeviceTables *t;
     int rc = device_tables(op, &t);
     if (rc) return rc;

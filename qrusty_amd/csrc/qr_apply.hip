@@ -50,6 +50,7 @@
 #include "qr_internal.h"
 
 #include <cstdlib>
+#include <mutex>
 #include <type_traits>
 
 namespace qr {
@@ -596,11 +597,21 @@ constexpr uint32_t kStageTileBytes = 16384;  // one tile: 2^11 complex64 or 2^10
 // filled from the one being read, does not wait for the load before every
 // later LDS read; the caller waits for it (vmcnt) and synchronises the block
 // before reading what it wrote.  It has no register result.
+// M0 is a reserved register that the compiler may hold a value in across the
+// statement, so the asm saves it in an SGPR of its own and restores it after
+// the load is issued (the load reads M0 at issue: back-to-back loads with
+// different M0 values each land at their own address).
 __device__ __forceinline__ void lds_dma16(const void *src, uint32_t lds) {
-    asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off"
-                 :
-                 : "v"(src), "s"(lds)
-                 : "memory", "m0");
+    uint32_t saved;
+    asm volatile(
+        "s_mov_b32 %0, m0\n\t"
+        "s_mov_b32 m0, %2\n\t"
+        "s_nop 0\n\t"
+        "global_load_lds_dwordx4 %1, off\n\t"
+        "s_mov_b32 m0, %0"
+        : "=&s"(saved)
+        : "v"(src), "s"(lds)
+        : "memory");
 }
 
 typedef __attribute__((address_space(3))) unsigned char lds_byte;
@@ -873,6 +884,32 @@ static bool prefer_staged(const qr_op *op, int L, size_t elem_bytes) {
     return big ? 2 * far >= 3 * distinct : far >= 2 * distinct;
 }
 
+// One memory pool per device for the per-launch <x|y> workspaces.  Its
+// release threshold keeps freed blocks in the pool, so that after the first
+// launch taking a buffer costs no device allocation.
+static std::mutex g_pool_mu;
+static hipMemPool_t g_pool[kMaxDevices];
+
+static int partials_pool(hipMemPool_t *out) {
+    int dev = 0;
+    QR_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(QR_ERR_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_pool_mu);
+    if (!g_pool[dev]) {
+        hipMemPoolProps props = {};
+        props.allocType = hipMemAllocationTypePinned;
+        props.location.type = hipMemLocationTypeDevice;
+        props.location.id = dev;
+        hipMemPool_t p;
+        QR_HIP_CHECK(hipMemPoolCreate(&p, &props));
+        uint64_t keep = ~uint64_t(0);
+        QR_HIP_CHECK(hipMemPoolSetAttribute(p, hipMemPoolAttrReleaseThreshold, &keep));
+        g_pool[dev] = p;
+    }
+    *out = g_pool[dev];
+    return QR_OK;
+}
+
 int launch_pauli_apply(const qr_op *op, DeviceTables *t, int32_t precision, uint32_t nvec,
                        const void *d_x, void *d_y, double *d_expect, hipStream_t s) {
     // QR_APPLY_WIDE=1 runs the 64-bit-mask instances on any operator, so that
@@ -896,17 +933,15 @@ int launch_pauli_apply(const qr_op *op, DeviceTables *t, int32_t precision, uint
     if ((N >> L) * uint64_t(nvec) > 0x7FFFFFFFull || nvec == 0)
         return fail(QR_ERR_SIZE, "apply: state too large for one launch (or nvec == 0)");
     const uint32_t ntiles = uint32_t(N >> L);
+    // <x|y> block partials: a buffer of this launch's own, taken from the
+    // device's pool and given back after the reduction, both in stream order,
+    // so launches on one operator from several streams never share one
     double *partials = nullptr;
     if (d_expect) {
-        const size_t need = size_t(ntiles) * nvec * 2;
-        if (t->partials_cap < need) {
-            if (t->partials) (void)hipFree(t->partials);
-            t->partials = nullptr;
-            t->partials_cap = 0;
-            QR_HIP_CHECK(hipMalloc(&t->partials, need * sizeof(double)));
-            t->partials_cap = need;
-        }
-        partials = t->partials;
+        hipMemPool_t pool;
+        if ((rc = partials_pool(&pool))) return rc;
+        QR_HIP_CHECK(hipMallocFromPoolAsync(reinterpret_cast<void **>(&partials),
+                                            size_t(ntiles) * nvec * 2 * sizeof(double), pool, s));
     }
     if (staged) {
         if (mixed) rc = narrow ? launch_staged<true, true>(op, t, nvec, d_x, d_y, partials, ntiles, s)
@@ -917,12 +952,17 @@ int launch_pauli_apply(const qr_op *op, DeviceTables *t, int32_t precision, uint
                                   : launch_tiled<true, false>(op, t, nvec, d_x, d_y, partials, ntiles, s);
     else rc = narrow ? launch_tiled<false, true>(op, t, nvec, d_x, d_y, partials, ntiles, s)
                      : launch_tiled<false, false>(op, t, nvec, d_x, d_y, partials, ntiles, s);
-    if (rc) return rc;
-    if (d_expect) {
+    if (!rc && d_expect) {
         hipLaunchKernelGGL(k_reduce_partials, dim3(nvec), dim3(256), 0, s, ntiles, partials,
                            d_expect);
-        QR_HIP_CHECK(hipGetLastError());
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) rc = hip_fail(e, "k_reduce_partials");
     }
+    if (partials) {
+        const hipError_t e = hipFreeAsync(partials, s);
+        if (!rc && e != hipSuccess) rc = hip_fail(e, "hipFreeAsync(partials)");
+    }
+    if (rc) return rc;
     return QR_OK;
 }
 

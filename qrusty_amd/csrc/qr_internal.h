@@ -102,8 +102,6 @@ struct DeviceTables {
     uint32_t *heavy = nullptr;   // [nheavy] heavy groups (qr_op::heavy)
     uint32_t nheavy = 0;
     uint32_t *leaf_heavy = nullptr;  // qr_op::leaf_heavy
-    double *partials = nullptr;  // workspace for <x|y> block partials
-    size_t partials_cap = 0;     // in doubles
     // matrix-free apply tables, one set per precision (tile size) and form
     // (slot = precision + 2 * staged)
     struct Patterns {

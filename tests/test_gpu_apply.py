@@ -140,6 +140,49 @@ def test_device_tensors_and_errors(hamiltonians):
         op.apply(x[:, :128].contiguous(), "fp64")
     with pytest.raises(qa.QrError):
         op.apply(x, "fp16")
+    # out: a host tensor, or one overlapping the state, is refused before any launch
+    with pytest.raises(qa.QrError):
+        op.apply(x, "fp64", out=torch.empty(2, 256, dtype=torch.complex128))
+    with pytest.raises(qa.QrError):
+        op.apply(x, "fp64", out=x)
+    big = torch.zeros(3, 256, dtype=torch.complex128, device="cuda")
+    big[:2] = x
+    with pytest.raises(qa.QrError):
+        op.apply(big[:2], "fp64", out=big[1:])
+
+
+def test_expectations_on_two_streams(hamiltonians):
+    """Launches of one operator on two streams at once each take their own
+    <x|y> workspace: both energies match the oracle."""
+    labels, coeffs = hamiltonians["py_H8"]
+    p = orc.Params(labels, coeffs)
+    op = _op(labels, coeffs)
+    xa = _states(p.N, 2, seed=41)
+    xb = _states(p.N, 2, seed=42)
+    ea_ref = orc.vdot(xa, orc.apply_full(p, xa))
+    eb_ref = orc.vdot(xb, orc.apply_full(p, xb))
+    import ctypes
+    from qrusty_amd import _lib as L
+    ta, tb = torch.from_numpy(xa).cuda(), torch.from_numpy(xb).cuda()
+    op.expectation(ta, "fp64")  # tables uploaded before the concurrent part
+    h = op._handle(L.QR_PHASE_ROWWISE)
+    torch.cuda.synchronize()
+    sa, sb = torch.cuda.Stream(), torch.cuda.Stream()
+    ea = [torch.empty(4, dtype=torch.float64, device="cuda") for _ in range(8)]
+    eb = [torch.empty(4, dtype=torch.float64, device="cuda") for _ in range(8)]
+    P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    for k in range(8):  # no synchronisation between the launches
+        L.check(L.lib.qr_pauli_apply(h.h, L.QR_PREC_FP64, 2, P(ta), None, P(ea[k]),
+                                     ctypes.c_void_p(sa.cuda_stream)))
+        L.check(L.lib.qr_pauli_apply(h.h, L.QR_PREC_FP64, 2, P(tb), None, P(eb[k]),
+                                     ctypes.c_void_p(sb.cuda_stream)))
+    torch.cuda.synchronize()
+    outs = {"a": [e.cpu().numpy().view(np.complex128) for e in ea],
+            "b": [e.cpu().numpy().view(np.complex128) for e in eb]}
+    for e in outs["a"]:
+        assert np.all(np.abs(e - ea_ref) <= 1e-12 * np.abs(ea_ref) + 1e-14)
+    for e in outs["b"]:
+        assert np.all(np.abs(e - eb_ref) <= 1e-12 * np.abs(eb_ref) + 1e-14)
 
 
 # The two kernel forms of the apply: "gather" (tile of x in LDS, partner tiles

@@ -232,3 +232,24 @@ def test_slice_repr(hamiltonians):
         H2[len(H2)]
     both = H2 + H2
     assert len(both) == 2 * len(H2)
+
+
+def test_apply_rejects_aliased_y_before_any_device_call():
+    # the overlap check runs before the library touches a device, so the
+    # (never dereferenced) addresses below only exercise the host check
+    from qrusty_amd import _lib as L
+    labels = (ctypes.c_char_p * 2)(b"XZI", b"ZZY")
+    coeffs = (ctypes.c_double * 4)(1.0, 0.0, 0.5, -0.25)
+    op = ctypes.c_void_p()
+    L.check(L.lib.qr_op_create(labels, coeffs, 2, L.QR_PHASE_ROWWISE, ctypes.byref(op)))
+    try:
+        base = 0x7F0000000000
+        for prec, elem in ((L.QR_PREC_FP64, 16), (L.QR_PREC_MIXED, 8)):
+            nbytes = 2 * 8 * elem  # 2 vectors of 2^3 rows
+            for dy in (0, elem, nbytes - elem, -(nbytes - elem)):
+                rc = L.lib.qr_pauli_apply(op, prec, 2, ctypes.c_void_p(base),
+                                          ctypes.c_void_p(base + dy), None, None)
+                assert rc == -1, (prec, dy)
+                assert b"overlaps" in L.lib.qr_last_error()
+    finally:
+        L.lib.qr_op_destroy(op)
