@@ -293,6 +293,24 @@ enum qr_precision { QR_PREC_FP64 = 0, QR_PREC_MIXED = 1 };
 int qr_pauli_apply(const qr_op *op, int32_t precision, uint32_t nvec, const void *d_x,
                    void *d_y, double *d_expect, void *stream);
 
+/* Allocation-free qr_pauli_apply.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory -- code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc.  Same arguments and
+ * results as qr_pauli_apply, but it calls no hipMalloc, hipFree or allocating
+ * library routine:
+ *  - the <x|y> block partials live in a file-scope device array of 4 MiB per
+ *    device (2^19 doubles): 2 * nvec * (2^n >> L) doubles must fit, with
+ *    L = 11 (QR_PREC_MIXED) or 10 (QR_PREC_FP64) -- e.g. 8 states of 24
+ *    qubits, or one of 28 -- else QR_ERR_SIZE;
+ *  - the operator's device tables for this precision must already be on the
+ *    device: one qr_pauli_apply call of the same operator and precision on
+ *    that device uploads them (QR_ERR_ARG otherwise).
+ * Every call on one device shares that array, which keeps the last call's
+ * contents: calls that overlap on one device (other streams or threads)
+ * race, so the caller serialises them. */
+int qr_pauli_apply_noalloc(const qr_op *op, int32_t precision, uint32_t nvec, const void *d_x,
+                           void *d_y, double *d_expect, void *stream);
+
 /* ------------------------------------------------------------------------
  * Batches (no counterpart in the reference, which builds and applies one
  * operator at a time: lib.rs:573-585, accel.rs:338-370).  One call builds the

@@ -12,6 +12,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libqrhip.so")
 
 QR_OK = 0
+QR_ERR_ARG = -1
+QR_ERR_SIZE = -5
 QR_PHASE_ROWWISE = 0
 QR_PHASE_KRON = 1
 QR_PREC_FP64 = 0
@@ -47,6 +49,7 @@ SIGNATURES = {
     "qr_csr_build_rows": (ctypes.c_int, [vp, u64, u64, vp, vp, vp]),
     "qr_csr_matvec": (ctypes.c_int, [u64, u64, u64, vp, vp, vp, vp, vp, vp]),
     "qr_pauli_apply": (ctypes.c_int, [vp, i32, u32, vp, vp, vp, vp]),
+    "qr_pauli_apply_noalloc": (ctypes.c_int, [vp, i32, u32, vp, vp, vp, vp]),
     "qr_csr_diagonal": (ctypes.c_int, [u64, u64, u64, vp, vp, vp, vp, vp]),
     "qr_axpby": (ctypes.c_int, [u64, f64p, vp, f64p, vp, vp, vp]),
     "qr_axpy": (ctypes.c_int, [u64, f64p, vp, vp, vp, vp]),

@@ -249,12 +249,15 @@ static int upload(T **dst, const std::vector<T> &src) {
     return QR_OK;
 }
 
-int device_tables(const qr_op *op, DeviceTables **out) {
+int device_tables(const qr_op *op, DeviceTables **out, bool noalloc) {
     int dev = 0;
     QR_HIP_CHECK(hipGetDevice(&dev));
     if (dev < 0 || dev >= kMaxDevices) return fail(QR_ERR_ARG, "device index out of range");
     std::lock_guard<std::mutex> lock(op->mu);
     DeviceTables &t = op->dev[dev];
+    if (!t.ready && noalloc)
+        return fail(QR_ERR_ARG, "apply_noalloc: the operator's tables are not on this device "
+                                "(one qr_pauli_apply of it on this device uploads them)");
     if (!t.ready) {
         int rc;
         std::vector<double2> tc2(op->T);
@@ -276,10 +279,14 @@ int device_tables(const qr_op *op, DeviceTables **out) {
 }
 
 int device_patterns(const qr_op *op, DeviceTables *t, int slot, int L, int tb, uint32_t pcap,
-                    uint32_t ucap, bool staged, bool pairs) {
+                    uint32_t ucap, bool staged, bool pairs, bool noalloc) {
     std::lock_guard<std::mutex> lock(op->mu);
     DeviceTables::Patterns &pt = t->pat[slot];
     if (pt.L == L) return QR_OK;
+    if (noalloc)
+        return fail(QR_ERR_ARG, "apply_noalloc: the operator's apply tables for this precision "
+                                "are not on this device (one qr_pauli_apply of it with this "
+                                "precision on this device uploads them)");
     for (void *p : {(void *)pt.grp, (void *)pt.chunk, (void *)pt.run, (void *)pt.ug, (void *)pt.pzl,
                     (void *)pt.ptptr, (void *)pt.ptz, (void *)pt.ptc})
         if (p) (void)hipFree(p);
@@ -751,8 +758,8 @@ int qr_mm_read_close(qr_mm_reader *r) {
     return QR_OK;
 }
 
-int qr_pauli_apply(const qr_op *op, int32_t precision, uint32_t nvec, const void *d_x, void *d_y,
-                   double *d_expect, void *stream) {
+static int apply_args(const qr_op *op, int32_t precision, uint32_t nvec, const void *d_x,
+                      void *d_y, double *d_expect) {
     if (!op || !d_x || (!d_y && !d_expect)) return fail(QR_ERR_ARG, "null argument");
     if (precision != QR_PREC_FP64 && precision != QR_PREC_MIXED)
         return fail(QR_ERR_ARG, "unknown precision");
@@ -765,10 +772,26 @@ int qr_pauli_apply(const qr_op *op, int32_t precision, uint32_t nvec, const void
         if (x0 < y0 + bytes && y0 < x0 + bytes)
             return fail(QR_ERR_ARG, "apply: y overlaps x (the apply is not in place)");
     }
+    return QR_OK;
+}
+
+int qr_pauli_apply(const qr_op *op, int32_t precision, uint32_t nvec, const void *d_x, void *d_y,
+                   double *d_expect, void *stream) {
+    int rc = apply_args(op, precision, nvec, d_x, d_y, d_expect);
+    if (rc || nvec == 0) return rc;
     DeviceTables *t;
-    int rc = device_tables(op, &t);
-    if (rc) return rc;
+    if ((rc = device_tables(op, &t))) return rc;
     return launch_pauli_apply(op, t, precision, nvec, d_x, d_y, d_expect, (hipStream_t)stream);
+}
+
+int qr_pauli_apply_noalloc(const qr_op *op, int32_t precision, uint32_t nvec, const void *d_x,
+                           void *d_y, double *d_expect, void *stream) {
+    int rc = apply_args(op, precision, nvec, d_x, d_y, d_expect);
+    if (rc || nvec == 0) return rc;
+    DeviceTables *t;
+    if ((rc = device_tables(op, &t, /*noalloc=*/true))) return rc;
+    return launch_pauli_apply(op, t, precision, nvec, d_x, d_y, d_expect, (hipStream_t)stream,
+                              /*noalloc=*/true);
 }
 
 }  // extern "C"

@@ -51,6 +51,7 @@
 
 #include <cstdlib>
 #include <mutex>
+#include <string>
 #include <type_traits>
 
 namespace qr {
@@ -910,8 +911,36 @@ static int partials_pool(hipMemPool_t *out) {
     return QR_OK;
 }
 
+// Workspace of the allocation-free entry point (qr_pauli_apply_noalloc): the
+// <x|y> block partials, 2 doubles per (tile, vector), in a file-scope device
+// array of 4 MiB that each device holds until the process ends.  Every call
+// on one device shares it and it keeps the last call's contents; each block
+// overwrites its own two slots, so nothing needs clearing.  In host code the
+// array's name is no device address: its address on each device comes from
+// hipGetSymbolAddress, taken once per device with that device current.
+constexpr size_t kScratchDoubles = size_t(1) << 19;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+static std::mutex g_scratch_mu;
+static double *g_scratch[kMaxDevices];
+
+static int scratch_partials(double **out) {
+    int dev = 0;
+    QR_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(QR_ERR_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        QR_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(QR_ERR_HIP, "hipGetSymbolAddress(scratch) returned no address");
+        g_scratch[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch[dev];
+    return QR_OK;
+}
+
 int launch_pauli_apply(const qr_op *op, DeviceTables *t, int32_t precision, uint32_t nvec,
-                       const void *d_x, void *d_y, double *d_expect, hipStream_t s) {
+                       const void *d_x, void *d_y, double *d_expect, hipStream_t s,
+                       bool noalloc) {
     // QR_APPLY_WIDE=1 runs the 64-bit-mask instances on any operator, so that
     // the tests can check them without a state of more than 32 qubits
     const char *wide_env = getenv("QR_APPLY_WIDE");
@@ -927,7 +956,7 @@ int launch_pauli_apply(const qr_op *op, DeviceTables *t, int32_t precision, uint
     // the gather form on full complex64 tiles runs the paired-row layout
     const bool pairs = mixed && !staged && L == Lmax;
     int rc = device_patterns(op, t, (mixed ? 1 : 0) + (staged ? 2 : 0), L, kApplyThreadBits, pcap,
-                             ucap, staged, pairs);
+                             ucap, staged, pairs, noalloc);
     if (rc) return rc;
     const uint64_t N = uint64_t(1) << op->nq;
     if ((N >> L) * uint64_t(nvec) > 0x7FFFFFFFull || nvec == 0)
@@ -937,7 +966,16 @@ int launch_pauli_apply(const qr_op *op, DeviceTables *t, int32_t precision, uint
     // device's pool and given back after the reduction, both in stream order,
     // so launches on one operator from several streams never share one
     double *partials = nullptr;
-    if (d_expect) {
+    if (d_expect && noalloc) {
+        const uint64_t need = uint64_t(ntiles) * nvec * 2;
+        if (need > kScratchDoubles)
+            return fail(QR_ERR_SIZE, "apply_noalloc: <x|H|x> of " + std::to_string(nvec) +
+                                         " vectors of " + std::to_string(ntiles) +
+                                         " tiles needs " + std::to_string(need * 8) +
+                                         " bytes of partials; the scratch array holds " +
+                                         std::to_string(kScratchDoubles * 8));
+        if ((rc = scratch_partials(&partials))) return rc;
+    } else if (d_expect) {
         hipMemPool_t pool;
         if ((rc = partials_pool(&pool))) return rc;
         QR_HIP_CHECK(hipMallocFromPoolAsync(reinterpret_cast<void **>(&partials),
@@ -958,7 +996,7 @@ int launch_pauli_apply(const qr_op *op, DeviceTables *t, int32_t precision, uint
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) rc = hip_fail(e, "k_reduce_partials");
     }
-    if (partials) {
+    if (partials && !noalloc) {
         const hipError_t e = hipFreeAsync(partials, s);
         if (!rc && e != hipSuccess) rc = hip_fail(e, "hipFreeAsync(partials)");
     }

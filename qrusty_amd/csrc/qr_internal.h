@@ -143,8 +143,10 @@ inline uint64_t pair_row_bits(uint64_t zl, int tb) {
 }
 void build_patterns(const qr_op *op, int L, int tb, uint32_t pcap, uint32_t ucap, bool staged,
                     bool pairs, HostPatterns &out);
+// With `noalloc` the tables must already be on the device (QR_ERR_ARG if not):
+// nothing is allocated or uploaded.
 int device_patterns(const qr_op *op, DeviceTables *t, int slot, int L, int tb, uint32_t pcap,
-                    uint32_t ucap, bool staged, bool pairs);
+                    uint32_t ucap, bool staged, bool pairs, bool noalloc = false);
 
 }  // namespace qr
 
@@ -182,7 +184,7 @@ struct qr_op {
 namespace qr {
 
 // Uploads the op's tables to the current device if needed; returns the tables.
-int device_tables(const qr_op *op, DeviceTables **out);
+int device_tables(const qr_op *op, DeviceTables **out, bool noalloc = false);
 
 // Kernel launchers (qr_csr.hip, qr_apply.hip).
 int launch_csr_build(const qr_op *op, const DeviceTables *t, uint64_t row_begin,
@@ -218,7 +220,10 @@ int io_read_raw(const char *path, uint64_t rows, uint64_t nnz, uint64_t *ip, uin
 int io_write_mm(const char *path, uint64_t rows, uint64_t cols, uint64_t nnz, const uint64_t *ip,
                 const uint64_t *ix, const double *d, int sym, hipStream_t s);
 int io_read_mm(const char *path, qr_mm_reader **out, hipStream_t s);
+// noalloc: no device allocation -- the tables must be on the device already
+// and the <x|y> block partials go to the file-scope scratch array (qr_apply.hip)
 int launch_pauli_apply(const qr_op *op, DeviceTables *t, int32_t precision, uint32_t nvec,
-                       const void *d_x, void *d_y, double *d_expect, hipStream_t s);
+                       const void *d_x, void *d_y, double *d_expect, hipStream_t s,
+                       bool noalloc = false);
 
 }  // namespace qr
