@@ -424,6 +424,106 @@ def bench_large_apply(args, torch, qa):
     return res
 
 
+def bench_giant(args, torch, qa):
+    """The reference's giant Hamiltonians, the operators its timing script
+    builds with RowwiseUnsafeChunked/1000 (pyqrusty/tests/perf_giant.py:33-40;
+    data from pyqrusty/tests/H_fixtures.py in tests/golden): the whole CSR
+    build where it fits in device memory (H10 64 GB, HJ 90 GB), else every row
+    built through a reused window of about 16 GB (H11 382 GB, H12 327 GB);
+    the matrix-free apply of each to one state (mixed and FP64); the CPU port
+    of RowwiseUnsafeChunked/1000 on a bounded sample of rows."""
+    from oracle import oracle as orc
+    from qrusty_amd import _lib as L
+    with open(os.path.join(HERE, "tests", "golden", "hamiltonians.json")) as f:
+        ham = json.load(f)
+    stream = torch.cuda.current_stream()
+    sh = ctypes.c_void_p(stream.cuda_stream)
+    P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    nthr = cpu_info()["threads"]
+    out = {}
+
+    def timed(fn, reps):
+        fn()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        a.record(stream)
+        for _ in range(reps):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+    for name in ("H10", "H11", "H12", "HJ"):
+        h_ = ham[f"py_{name}"]
+        labels = h_["labels"]
+        coeffs = [complex(a, b) for a, b in h_["coeffs"]]
+        op = qa.SparsePauliOp([qa.Pauli(l) for l in labels], coeffs)
+        h = op._handle(L.QR_PHASE_ROWWISE)
+        nq, G = h.num_qubits, h.G
+        N = 1 << nq
+        nnz = N * G
+        res = {"num_qubits": nq, "terms": h.T, "groups": G, "nnz": nnz,
+               "csr_GB": 24 * nnz / 1e9}
+        window = 16 << 30
+        if 24 * nnz + 8 * (N + 1) <= 120 << 30:
+            ip = torch.empty(N + 1, dtype=torch.int64, device="cuda")
+            ix = torch.empty(nnz, dtype=torch.int64, device="cuda")
+            dv = torch.empty(nnz, dtype=torch.complex128, device="cuda")
+            ms = timed(lambda: L.check(L.lib.qr_csr_build(h.h, P(ip), P(ix), P(dv), sh)), 3)
+            res["csr_build"] = "whole"
+            del ip, ix, dv
+        else:
+            W = max(16, (window // (24 * G)) // 16 * 16)
+            ix = torch.empty(W * G, dtype=torch.int64, device="cuda")
+            dv = torch.empty(W * G, dtype=torch.complex128, device="cuda")
+
+            def sweep():
+                for r0 in range(0, N, W):
+                    L.check(L.lib.qr_csr_build_rows(h.h, r0, min(N, r0 + W), P(ix), P(dv), sh))
+            ms = timed(sweep, 2)
+            res["csr_build"] = f"every row through a window of {W} rows ({24 * W * G / 2**30:.1f} GiB)"
+            del ix, dv
+        torch.cuda.empty_cache()
+        res.update({"build_ms": ms, "build_nnz_per_s": nnz / (ms / 1e3),
+                    "build_GBps": (24 * nnz + 8 * (N + 1)) / ms / 1e6})
+        res["build_pct_hbm"] = 100 * res["build_GBps"] / HBM_PEAK_GBPS
+        g = torch.Generator(device="cuda")
+        g.manual_seed(23)
+        x64 = torch.randn(N, dtype=torch.complex128, device="cuda", generator=g)
+        x64 /= torch.linalg.vector_norm(x64)
+        x32 = x64.to(torch.complex64)
+        y64, y32 = torch.empty_like(x64), torch.empty_like(x32)
+        e = torch.empty(2, dtype=torch.float64, device="cuda")
+        for prec, xt, yt, key, sz in ((L.QR_PREC_MIXED, x32, y32, "mixed", 8),
+                                      (L.QR_PREC_FP64, x64, y64, "fp64", 16)):
+            ms_a = timed(lambda: L.check(L.lib.qr_pauli_apply(h.h, prec, 1, P(xt), P(yt), P(e), sh)), 3)
+            res[f"apply_{key}_ms"] = ms_a
+            res[f"apply_{key}_GBps"] = 2 * sz * N / ms_a / 1e6
+        res["apply_mixed_rel_err_y"] = (torch.linalg.vector_norm(y32.to(torch.complex128) - y64) /
+                                        torch.linalg.vector_norm(y64)).item()
+        del x64, x32, y64, y32
+        torch.cuda.empty_cache()
+        # CPU port of RowwiseUnsafeChunked/1000 on the first rows
+        p = orc.Params(labels, coeffs)
+        rows = 1000
+        t0 = time.perf_counter()
+        orc.build_csr(p, 0, rows, step=1000, nthreads=nthr)
+        t1 = time.perf_counter() - t0
+        rows = int(min(N, max(1000, rows * 2.0 / max(t1, 1e-3)))) // 1000 * 1000 or 1000
+        t0 = time.perf_counter()
+        _, ind, _ = orc.build_csr(p, 0, rows, step=1000, nthreads=nthr)
+        tc = time.perf_counter() - t0
+        res["cpu_port_build_nnz_per_s"] = len(ind) / tc
+        res["cpu_port_sample"] = f"rows 0..{rows} of 2^{nq}, {nthr} threads, one run"
+        res["gpu_over_cpu_build"] = res["build_nnz_per_s"] / res["cpu_port_build_nnz_per_s"]
+        del ind
+        out[name] = res
+        log(f"giant {name}: n={nq} T={h.T} G={G} build {ms:.2f} ms ({res['build_GBps']:.0f} GB/s, "
+            f"{res['build_nnz_per_s']:.3e} nnz/s; CPU port {res['cpu_port_build_nnz_per_s']:.3e}), "
+            f"apply mixed {res['apply_mixed_ms']:.3f} ms, fp64 {res['apply_fp64_ms']:.3f} ms")
+    return out
+
+
 def bench_vec(args, torch, qa):
     """Eigensolver-step operations on the device: axpby and precond2 on
     2^27-element complex128 vectors (48 B per element: two reads, one write),
@@ -616,7 +716,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the Heisenberg apply workload")
     ap.add_argument("--no-check", action="store_true", help="skip the sampled parity check")
-    ap.add_argument("--only", choices=["csr", "apply", "batch", "large", "vec", "io", "copy"],
+    ap.add_argument("--only", choices=["csr", "apply", "batch", "large", "giant", "vec", "io", "copy"],
                     default=None, help="one section; io (files) runs only when named")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -660,6 +760,8 @@ def main():
         extra["batch"] = bench_batch(args, torch, qa)
     if args.only in (None, "large") and not args.no_extra:
         extra["large_apply"] = bench_large_apply(args, torch, qa)
+    if args.only in (None, "giant") and not args.no_extra:
+        extra["giant"] = bench_giant(args, torch, qa)
     if args.only in (None, "vec") and not args.no_extra:
         extra["vec"] = bench_vec(args, torch, qa)
     if args.only == "io":

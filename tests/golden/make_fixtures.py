@@ -6,7 +6,8 @@ as TEXT and parsed with regular expressions -- nothing from them is imported
 or executed:
 
 * pyqrusty/tests/H_fixtures.py: ``NAME = build_spop([labels...], coeffs=[...])``
-  (H2 4q/27, H4 8q/361, H6 12q/1819, H8 16q/5793, H12 24q/4497, HJ 27q/55)
+  (H2 4q/27, H4 8q/361, H6 12q/1819, H8 16q/5793, H10 20q/14251, H11 22q/20918,
+  H12 24q/4497, HJ 27q/55; the timed operators of pyqrusty/tests/perf_giant.py:33-40)
 * qrusty/src/fixtures.rs: ``TestCase`` H2/H4/H6 with full-precision coefficient
   strings (lib.rs tests h2/h4/h6 use these).
 
@@ -40,7 +41,7 @@ sys.path.insert(0, REPO)
 from oracle import oracle as orc  # noqa: E402
 from qrusty_amd import workloads  # noqa: E402
 
-PY_FIXTURES = ["H2", "H4", "H6", "H8", "H12", "HJ"]
+PY_FIXTURES = ["H2", "H4", "H6", "H8", "H10", "H11", "H12", "HJ"]
 RS_FIXTURES = ["H2", "H4", "H6"]
 
 
