@@ -227,9 +227,16 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
         const uint32_t j = (uint32_t(c) << jw_log2) + lane;
         rfx[c] = int32_t(kSub ? op.gbase + j : j);
         hs[c] = -1;
-        if (kHeavy)
-            for (uint32_t h = 0; h < op.nheavy; ++h)
-                if (op.heavy[h] == j) hs[c] = int32_t(h);
+        if (kHeavy) {
+            // heavy groups are listed ascending: binary search
+            uint32_t lo = 0, hi = op.nheavy;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (op.heavy[mid] < j) lo = mid + 1;
+                else hi = mid;
+            }
+            if (lo < op.nheavy && op.heavy[lo] == j) hs[c] = int32_t(lo);
+        }
         int32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
         s01[c] = s23[c] = 0;
         xj[c] = z0[c] = z1[c] = 0;
