@@ -99,6 +99,26 @@ int qr_op_create(const char *const *labels, const double *coeffs /* 2*num_terms 
                  uint64_t num_terms, int32_t phase_convention, qr_op **out);
 int qr_op_destroy(qr_op *op);
 
+/* qr_op_create with the term tables built on the current device, for
+ * operators with many terms: the labels are packed as bytes and parsed there
+ * (one thread per label), the terms grouped by x mask with a stable radix sort
+ * (wavefront ballots rank equal digits) and a segmented scan, and the rank
+ * lists built one thread per group.  The result -- tables, errors and their
+ * messages, including which failing term is reported -- is identical to
+ * qr_op_create's; the device tables of the current device are already in
+ * place.  At most 2^28 terms (QR_ERR_SIZE).  Synchronises the stream. */
+int qr_op_create_device(const char *const *labels, const double *coeffs /* 2*num_terms */,
+                        uint64_t num_terms, int32_t phase_convention, void *stream, qr_op **out);
+
+/* The grouped tables (host copies), for inspection: gx[G] (distinct x masks,
+ * ascending), gterm[G+1] (term range of each group), tz[T] and tc[2T] (z masks
+ * and c' in grouped order, input order inside a group), gkind[G] (0 complex,
+ * 1 real, 2 imaginary coefficients), rl_ptr[G+1] and rl[2E] ((bit, sigma)
+ * rank-list entries).  Any output may be NULL; *num_rank_entries = E.
+ * Host only. */
+int qr_op_tables(const qr_op *op, uint64_t *gx, uint32_t *gterm, uint64_t *tz, double *tc,
+                 uint8_t *gkind, uint32_t *rl_ptr, int32_t *rl, uint64_t *num_rank_entries);
+
 /* num_qubits n, terms T, distinct x masks G, CSR nnz = 2^n * G (every row of
  * the rowwise CSR holds exactly G entries, explicit zeros kept; accel.rs:191-208).
  * nnz is 0 when 2^n * G overflows 64 bits. */

@@ -133,13 +133,23 @@ class Pauli:
 # ---------------------------------------------------------------------------
 # device operator handle
 # ---------------------------------------------------------------------------
+# Operators with at least this many terms build their term tables on the
+# device (qr_op_create_device) when one is present; the tables are identical.
+DEVICE_GROUPING_MIN_TERMS = 10_000
+
+
 class _OpHandle:
     def __init__(self, labels, coeffs, convention):
         arr = (ctypes.c_char_p * len(labels))(*[s.encode() for s in labels])
         c = np.ascontiguousarray(np.asarray(coeffs, dtype=np.complex128)).view(np.float64)
         h = ctypes.c_void_p()
-        L.check(L.lib.qr_op_create(arr, c.ctypes.data_as(L.f64p), len(labels), convention,
-                                   ctypes.byref(h)))
+        torch = _torch() if len(labels) >= DEVICE_GROUPING_MIN_TERMS else None
+        if torch is not None and torch.cuda.is_available():
+            L.check(L.lib.qr_op_create_device(arr, c.ctypes.data_as(L.f64p), len(labels), convention,
+                                              _stream_handle(torch), ctypes.byref(h)))
+        else:
+            L.check(L.lib.qr_op_create(arr, c.ctypes.data_as(L.f64p), len(labels), convention,
+                                       ctypes.byref(h)))
         self.h = h
         nq, T, G, nnz = ctypes.c_uint32(), ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
         L.check(L.lib.qr_op_info(h, ctypes.byref(nq), ctypes.byref(T), ctypes.byref(G),

@@ -41,6 +41,10 @@ SIGNATURES = {
     "qr_op_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_char_p), f64p, u64, i32,
                                     ctypes.POINTER(vp)]),
     "qr_op_destroy": (ctypes.c_int, [vp]),
+    "qr_op_create_device": (ctypes.c_int, [ctypes.POINTER(ctypes.c_char_p), f64p, u64, i32, vp,
+                                           ctypes.POINTER(vp)]),
+    "qr_op_tables": (ctypes.c_int, [vp, u64p, u32p, u64p, f64p, ctypes.POINTER(ctypes.c_uint8),
+                                    u32p, i32p, u64p]),
     "qr_op_info": (ctypes.c_int, [vp, u32p, u64p, u64p, u64p]),
     "qr_op_terms": (ctypes.c_int, [vp, u64p, u64p, f64p]),
     "qr_op_row_layout": (ctypes.c_int, [vp, u64, u64p, u32p]),

@@ -16,8 +16,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libqrhip.so")
-SOURCES = ["qr_api.cpp", "qr_csr.hip", "qr_apply.hip", "qr_batch.hip", "qr_vec.hip",
-           "qr_spadd.hip", "qr_filter.hip", "qr_io.hip"]
+SOURCES = ["qr_api.cpp", "qr_csr.hip", "qr_apply.hip", "qr_batch.hip", "qr_group.hip",
+           "qr_vec.hip", "qr_spadd.hip", "qr_filter.hip", "qr_io.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -99,7 +99,7 @@ static void build_rank_lists(const std::vector<uint64_t> &gx, std::vector<uint32
     for (size_t j = 0; j < G; ++j) out.insert(out.end(), lists[j].begin(), lists[j].end());
 }
 
-static void free_tables(DeviceTables &t) {
+void free_tables(DeviceTables &t) {
     void *ptrs[] = {t.gx, t.gterm, t.tz, t.tc, t.rl_ptr, t.rl, t.gkind, t.heavy, t.leaf_heavy};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -313,6 +313,53 @@ int device_patterns(const qr_op *op, DeviceTables *t, int slot, int L, int tb, u
     return QR_OK;
 }
 
+// The parts of an operator derived from its grouped tables (gx, gterm):
+// the sub-operators (subtrees of at most kSubMaxG groups) and heavy groups.
+void finish_op(qr_op *op) {
+    // subtrees of at most kSubMaxG groups, split where build_rank_lists splits
+    if (op->G > kSubMaxG) {
+        struct Node { uint32_t lo, hi, depth; };
+        std::vector<Node> stack{{0, uint32_t(op->G), 0}};
+        while (!stack.empty()) {
+            const Node nd = stack.back();
+            stack.pop_back();
+            if (nd.hi - nd.lo <= kSubMaxG) {
+                op->leaves.push_back({nd.lo, nd.hi, nd.depth, 0, 0});
+                continue;
+            }
+            const uint64_t bit = uint64_t(1) << (63 - __builtin_clzll(op->gx[nd.lo] ^ op->gx[nd.hi - 1]));
+            const uint32_t m = uint32_t(std::partition_point(op->gx.begin() + nd.lo,
+                                                             op->gx.begin() + nd.hi,
+                                                             [bit](uint64_t v) { return (v & bit) == 0; }) -
+                                        op->gx.begin());
+            stack.push_back({m, nd.hi, nd.depth + 1});
+            stack.push_back({nd.lo, m, nd.depth + 1});
+        }
+    }
+    // heavy groups: at least 8 terms and 4x the mean, the largest kMaxHeavy
+    {
+        const double mean = double(op->T) / double(op->G);
+        std::vector<std::pair<uint32_t, uint32_t>> cand;  // (terms, group)
+        for (uint64_t g = 0; g < op->G; ++g) {
+            const uint32_t n = op->gterm[g + 1] - op->gterm[g];
+            if (n >= 8 && double(n) >= 4.0 * mean) cand.emplace_back(n, uint32_t(g));
+        }
+        std::sort(cand.begin(), cand.end(), [](const auto &a, const auto &b) {
+            return a.first != b.first ? a.first > b.first : a.second < b.second;
+        });
+        if (cand.size() > kMaxHeavy) cand.resize(kMaxHeavy);
+        for (const auto &c : cand) op->heavy.push_back(c.second);
+        std::sort(op->heavy.begin(), op->heavy.end());
+        // each sub-operator's heavy groups, in its own (leaf-local) indices
+        for (auto &lf : op->leaves) {
+            lf.hoff = uint32_t(op->leaf_heavy.size());
+            for (uint32_t g : op->heavy)
+                if (g >= lf.s0 && g < lf.s1) op->leaf_heavy.push_back(g - lf.s0);
+            lf.hcnt = uint32_t(op->leaf_heavy.size()) - lf.hoff;
+        }
+    }
+}
+
 }  // namespace qr
 
 using namespace qr;
@@ -431,49 +478,39 @@ int qr_op_create(const char *const *labels, const double *coeffs, uint64_t num_t
         op->gkind[g] = real ? 1 : (imag ? 2 : 0);
     }
     build_rank_lists(op->gx, op->rl_ptr, op->rl);
-    // subtrees of at most kSubMaxG groups, split where build_rank_lists splits
-    if (op->G > kSubMaxG) {
-        struct Node { uint32_t lo, hi, depth; };
-        std::vector<Node> stack{{0, uint32_t(op->G), 0}};
-        while (!stack.empty()) {
-            const Node nd = stack.back();
-            stack.pop_back();
-            if (nd.hi - nd.lo <= kSubMaxG) {
-                op->leaves.push_back({nd.lo, nd.hi, nd.depth, 0, 0});
-                continue;
-            }
-            const uint64_t bit = uint64_t(1) << (63 - __builtin_clzll(op->gx[nd.lo] ^ op->gx[nd.hi - 1]));
-            const uint32_t m = uint32_t(std::partition_point(op->gx.begin() + nd.lo,
-                                                             op->gx.begin() + nd.hi,
-                                                             [bit](uint64_t v) { return (v & bit) == 0; }) -
-                                        op->gx.begin());
-            stack.push_back({m, nd.hi, nd.depth + 1});
-            stack.push_back({nd.lo, m, nd.depth + 1});
-        }
-    }
-    // heavy groups: at least 8 terms and 4x the mean, the largest kMaxHeavy
-    {
-        const double mean = double(op->T) / double(op->G);
-        std::vector<std::pair<uint32_t, uint32_t>> cand;  // (terms, group)
-        for (uint64_t g = 0; g < op->G; ++g) {
-            const uint32_t n = op->gterm[g + 1] - op->gterm[g];
-            if (n >= 8 && double(n) >= 4.0 * mean) cand.emplace_back(n, uint32_t(g));
-        }
-        std::sort(cand.begin(), cand.end(), [](const auto &a, const auto &b) {
-            return a.first != b.first ? a.first > b.first : a.second < b.second;
-        });
-        if (cand.size() > kMaxHeavy) cand.resize(kMaxHeavy);
-        for (const auto &c : cand) op->heavy.push_back(c.second);
-        std::sort(op->heavy.begin(), op->heavy.end());
-        // each sub-operator's heavy groups, in its own (leaf-local) indices
-        for (auto &lf : op->leaves) {
-            lf.hoff = uint32_t(op->leaf_heavy.size());
-            for (uint32_t g : op->heavy)
-                if (g >= lf.s0 && g < lf.s1) op->leaf_heavy.push_back(g - lf.s0);
-            lf.hcnt = uint32_t(op->leaf_heavy.size()) - lf.hoff;
-        }
-    }
+    finish_op(op);
     *out = op;
+    return QR_OK;
+}
+
+int qr_op_create_device(const char *const *labels, const double *coeffs, uint64_t num_terms,
+                        int32_t phase_convention, void *stream, qr_op **out) {
+    if (!out) return fail(QR_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (num_terms == 0)
+        return fail(QR_ERR_TERMS, "SparsePauliOp::new: at least one pauli must be supplied");
+    if (!labels || !coeffs) return fail(QR_ERR_ARG, "null labels or coefficients");
+    if (phase_convention != QR_PHASE_ROWWISE && phase_convention != QR_PHASE_KRON)
+        return fail(QR_ERR_ARG, "unknown phase convention");
+    if (num_terms > 0xFFFFFFF0ull) return fail(QR_ERR_SIZE, "too many terms");
+    return group_terms_device(labels, coeffs, num_terms, phase_convention, (hipStream_t)stream, out);
+}
+
+int qr_op_tables(const qr_op *op, uint64_t *gx, uint32_t *gterm, uint64_t *tz, double *tc,
+                 uint8_t *gkind, uint32_t *rl_ptr, int32_t *rl, uint64_t *num_rank_entries) {
+    if (!op) return fail(QR_ERR_ARG, "null operator");
+    if (gx) std::copy(op->gx.begin(), op->gx.end(), gx);
+    if (gterm) std::copy(op->gterm.begin(), op->gterm.end(), gterm);
+    if (tz) std::copy(op->tz.begin(), op->tz.end(), tz);
+    if (tc) std::copy(op->tc.begin(), op->tc.end(), tc);
+    if (gkind) std::copy(op->gkind.begin(), op->gkind.end(), gkind);
+    if (rl_ptr) std::copy(op->rl_ptr.begin(), op->rl_ptr.end(), rl_ptr);
+    if (rl)
+        for (size_t e = 0; e < op->rl.size(); ++e) {
+            rl[2 * e] = op->rl[e].bit;
+            rl[2 * e + 1] = op->rl[e].sigma;
+        }
+    if (num_rank_entries) *num_rank_entries = op->rl.size();
     return QR_OK;
 }
 

@@ -37,7 +37,11 @@ int hip_fail(hipError_t e, const char *what);
     } while (0)
 
 constexpr int kMaxDevices = 16;
-constexpr uint32_t kMaxHeavy = 4;  // heavy groups the CSR build folds per row
+// heavy groups the CSR build folds per row in a pass of their own: molecular
+// Hamiltonians (the reference's H8..H12) have 17-111 groups of 22-301 terms
+// next to groups of at most 6 (as many as the tile's LDS holds are folded
+// that way, heavy_fit in qr_tile.h)
+constexpr uint32_t kMaxHeavy = 128;
 constexpr uint32_t kSubMaxG = 2048;  // groups of one CSR-build sub-operator
 // CSR matvecs load the CSR stream nontemporally from this many entries per
 // row on (measured crossover; qr_csr.hip)
@@ -182,6 +186,17 @@ struct qr_op {
 };
 
 namespace qr {
+
+// An operator whose term tables are built on the current device
+// (qr_group.hip): the same tables as qr_op_create's host path, and its device
+// tables for this device already in place.  Synchronises the stream.
+int group_terms_device(const char *const *labels, const double *coeffs, uint64_t T, int32_t conv,
+                       hipStream_t s, qr_op **out);
+
+// Sub-operators and heavy groups of an op whose grouped tables are set.
+void finish_op(qr_op *op);
+// Frees one device's tables of an op.
+void free_tables(DeviceTables &t);
 
 // Uploads the op's tables to the current device if needed; returns the tables.
 int device_tables(const qr_op *op, DeviceTables **out, bool noalloc = false);
