@@ -8,14 +8,18 @@ values) followed by one complex128 CSR matvec on a normalised random state.
 `value` is whole-step throughput in CSR nonzeros per second; inputs are
 resident in device memory when timing starts.
 
-Also reported (same JSON line, key "extra"): the 24-qubit Heisenberg ring
-matrix-free apply on 8 states, FP64 vs mixed precision (GB/s, % of HBM peak,
-speedup, error vs FP64), unless --no-extra.
+Also reported (same JSON line, key "extra"), unless --no-extra: the 24-qubit
+Heisenberg ring matrix-free apply on 8 states, FP64 vs mixed precision (GB/s,
+% of HBM peak, speedup, error vs FP64); the 256-operator uneven batch; the
+28-qubit k-local apply; the reference's giant Hamiltonians (H10, H11, H12, HJ);
+the eigensolver-step operations; the box's plain copy/fill/read rates.
 
 roofline: the dominant kernel's algorithmic bytes per launch / its mean
 duration from HIP events on the launch stream.  cpu_baseline: the oracle's C
 restatement of qrusty's RowwiseUnsafeChunked/1000 build + spmat_dot_densevec
-on a bounded sample of rows, on up to 16 host threads.
+on a bounded sample of rows, on every host core this process may use (capped
+at the CPU share the environment grants, OMP_NUM_THREADS); the core count, the
+CPU model and the sampled fraction are recorded.
 """
 from __future__ import annotations
 
@@ -196,6 +200,47 @@ def cpu_baseline_csr(labels, coeffs, target_s=4.0):
 
 
 # ---------------------------------------------------------------------------
+def bench_apply_one(args, torch, qa, precision):
+    """24-qubit Heisenberg ring, 8 states, one precision only: `steps` applies
+    with <psi|H|psi> after `warmup` untimed ones (a short program for profiler
+    counter passes: every dispatch of the apply kernel is the same call)."""
+    from qrusty_amd import _lib as L
+    from qrusty_amd import workloads
+    nq, nvec = 24, 8
+    labels, coeffs = workloads.heisenberg_ring(nq, seed=workloads.SEED_CFG3)
+    op = qa.SparsePauliOp([qa.Pauli(l) for l in labels], coeffs)
+    h = op._handle(L.QR_PHASE_ROWWISE)
+    N = 1 << nq
+    mixed = precision == "mixed"
+    dt = torch.complex64 if mixed else torch.complex128
+    prec = L.QR_PREC_MIXED if mixed else L.QR_PREC_FP64
+    g = torch.Generator(device="cuda")
+    g.manual_seed(workloads.SEED_CFG3 + 1)
+    x = torch.randn(nvec, N, dtype=dt, device="cuda", generator=g)
+    x /= torch.linalg.vector_norm(x, dim=1, keepdim=True)
+    y = torch.empty_like(x)
+    e = torch.empty(2 * nvec, dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream()
+    sh = ctypes.c_void_p(stream.cuda_stream)
+    P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    call = lambda: L.check(L.lib.qr_pauli_apply(h.h, prec, nvec, P(x), P(y), P(e), sh))  # noqa: E731
+    for _ in range(args.warmup):
+        call()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    a.record(stream)
+    for _ in range(args.steps):
+        call()
+    b.record(stream)
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / args.steps
+    byts = nvec * N * 2 * (8 if mixed else 16)
+    log(f"apply {precision}: {ms:.3f} ms ({byts / ms / 1e6:.0f} GB/s)")
+    return {"workload": f"heisenberg ring n=24, 8 states, {precision} apply + <psi|H|psi>",
+            "ms": ms, "GBps": byts / ms / 1e6, "pct_hbm": 100 * byts / ms / 1e6 / HBM_PEAK_GBPS,
+            "algorithmic_bytes": byts}
+
+
 def bench_apply(args, torch, qa):
     """24-qubit Heisenberg ring, 8 states: FP64 vs mixed matrix-free apply."""
     import numpy as np
@@ -432,6 +477,7 @@ def bench_giant(args, torch, qa):
     built through a reused window of about 16 GB (H11 382 GB, H12 327 GB);
     the matrix-free apply of each to one state (mixed and FP64); the CPU port
     of RowwiseUnsafeChunked/1000 on a bounded sample of rows."""
+    import numpy as np
     from oracle import oracle as orc
     from qrusty_amd import _lib as L
     with open(os.path.join(HERE, "tests", "golden", "hamiltonians.json")) as f:
@@ -464,6 +510,26 @@ def bench_giant(args, torch, qa):
         nnz = N * G
         res = {"num_qubits": nq, "terms": h.T, "groups": G, "nnz": nnz,
                "csr_GB": 24 * nnz / 1e9}
+        # operator tables: host path (qr_op_create) against the device path
+        # (qr_op_create_device), best of 3 after one untimed device call
+        arr = (ctypes.c_char_p * len(labels))(*[s_.encode() for s_ in labels])
+        cbuf = np.ascontiguousarray(np.asarray(coeffs, dtype=np.complex128)).view(np.float64)
+
+        def create(device):
+            hh = ctypes.c_void_p()
+            t0 = time.perf_counter()
+            if device:
+                L.check(L.lib.qr_op_create_device(arr, cbuf.ctypes.data_as(L.f64p), len(labels),
+                                                  L.QR_PHASE_ROWWISE, sh, ctypes.byref(hh)))
+            else:
+                L.check(L.lib.qr_op_create(arr, cbuf.ctypes.data_as(L.f64p), len(labels),
+                                           L.QR_PHASE_ROWWISE, ctypes.byref(hh)))
+            dt = time.perf_counter() - t0
+            L.lib.qr_op_destroy(hh)
+            return dt
+        create(True)
+        res["create_host_ms"] = 1e3 * min(create(False) for _ in range(3))
+        res["create_device_ms"] = 1e3 * min(create(True) for _ in range(3))
         window = 16 << 30
         if 24 * nnz + 8 * (N + 1) <= 120 << 30:
             ip = torch.empty(N + 1, dtype=torch.int64, device="cuda")
@@ -520,7 +586,8 @@ def bench_giant(args, torch, qa):
         out[name] = res
         log(f"giant {name}: n={nq} T={h.T} G={G} build {ms:.2f} ms ({res['build_GBps']:.0f} GB/s, "
             f"{res['build_nnz_per_s']:.3e} nnz/s; CPU port {res['cpu_port_build_nnz_per_s']:.3e}), "
-            f"apply mixed {res['apply_mixed_ms']:.3f} ms, fp64 {res['apply_fp64_ms']:.3f} ms")
+            f"apply mixed {res['apply_mixed_ms']:.3f} ms, fp64 {res['apply_fp64_ms']:.3f} ms; "
+            f"tables host {res['create_host_ms']:.2f} ms, device {res['create_device_ms']:.2f} ms")
     return out
 
 
@@ -716,7 +783,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the Heisenberg apply workload")
     ap.add_argument("--no-check", action="store_true", help="skip the sampled parity check")
-    ap.add_argument("--only", choices=["csr", "apply", "batch", "large", "giant", "vec", "io", "copy"],
+    ap.add_argument("--only", choices=["csr", "apply", "apply-mixed", "apply-fp64", "batch", "large",
+                                       "giant", "vec", "io", "copy"],
                     default=None, help="one section; io (files) runs only when named")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -750,6 +818,11 @@ def main():
             log("cpu baseline ...")
             out["cpu_baseline"] = cpu_baseline_csr(labels, coeffs)
             log(f"cpu baseline {out['cpu_baseline']['value']:.3e} nnz/s")
+    if args.only in ("apply-mixed", "apply-fp64"):
+        a = bench_apply_one(args, torch, qa, args.only.split("-")[1])
+        out.update({"value": a["GBps"], "unit": "GB/s", "ms_per_step": a["ms"],
+                    "config": {"workload": a["workload"]}})
+        extra[args.only] = a
     if args.only in (None, "apply") and not args.no_extra:
         extra["apply"] = bench_apply(args, torch, qa)
         if args.only == "apply":
