@@ -281,6 +281,39 @@ def test_matvec_short_rows_ragged():
         _check_matvec(y, ref, abs(A).dot(np.abs(x)))
 
 
+@pytest.mark.parametrize("rows,cols,unsorted", [(1, 3000, False), (7, 700, True), (9, 5000, False),
+                                                (33, 2048, True), (1001, 1500, False),
+                                                (4096, 1 << 14, False)])
+def test_matvec_long_rows_ragged(rows, cols, unsorted):
+    """256+ entries per row on average (the kernel that stages each wave's 8
+    rows in LDS): rows of 0 to 3000 entries side by side in one wave, chunk
+    tails, row counts that are not a multiple of 8 rows per wave or 32 per
+    block, unsorted columns and repeated columns within a row."""
+    import scipy.sparse as sps
+    import qrusty_amd as qa
+    rng = np.random.default_rng(rows * 7 + cols)
+    lens = rng.integers(256, 700, size=rows)
+    lens[rng.choice(rows, size=max(1, rows // 9), replace=False)] = rng.integers(0, 3001)
+    lens[rng.choice(rows, size=max(1, rows // 13), replace=False)] = 0
+    lens[rng.choice(rows, size=max(1, rows // 11), replace=False)] = 65
+    indptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    parts = []
+    for n in lens:
+        c = rng.integers(0, cols, size=int(n))
+        parts.append(c if unsorted else np.sort(c))
+    indices = np.concatenate(parts + [np.zeros(0, dtype=np.int64)]).astype(np.int64)
+    data = rng.standard_normal(indptr[-1]) + 1j * rng.standard_normal(indptr[-1])
+    avg = indptr[-1] // rows
+    print(f"[gpu] long-row matvec rows={rows} cols={cols} avg={avg} unsorted={unsorted}", flush=True)
+    assert avg >= 256 or rows < 8
+    m = qa.SpMat.new_unchecked((rows, cols), data, indices, indptr)
+    x = rng.standard_normal(cols) + 1j * rng.standard_normal(cols)
+    y = qa.spmat_dot_densevec(m, x)
+    ref = orc.spmat_dot_densevec(indptr.astype(np.uint64), indices.astype(np.uint64), data, x)
+    A = sps.csr_matrix((np.abs(data), indices, indptr), shape=(rows, cols))
+    _check_matvec(y, ref, A.dot(np.abs(x)))
+
+
 def test_device_tensor_matvec_stays_on_device(hamiltonians):
     import qrusty_amd as qa
     m = _op(*hamiltonians["py_H4"]).to_matrix_mode("Hip")
