@@ -76,10 +76,14 @@ __global__ __launch_bounds__(kStageThreads, QR_BATCH_WAVES(kMatvec, kChunks)) vo
 }
 
 // Split form, second pass: y_k = A_k x_k from the CSR just written, over
-// (operator, block of kMvRows rows) items taken from one counter.  Eight lanes
-// share a row and a 256-thread block covers 32 consecutive rows per step, so
-// the gathers x[r ^ x_g] of neighbouring rows share cache lines (as in
-// k_csr_matvec).  Every row of a batch operator holds exactly G entries.
+// (operator, block of kMvRows rows) items taken from one counter.  Operators
+// of at least kMvStagedMinRow groups run the row-staged walker of qr_mv.h
+// (contiguous row streams, gathers of 8 neighbouring rows per instruction);
+// shorter rows 8 lanes per row, a 256-thread block covering 32 consecutive
+// rows per step, so that the gathers x[r ^ x_g] of neighbouring rows share
+// cache lines.  Every row of a batch operator holds exactly G entries.  The
+// arrays come from descriptors in memory, so they are read through pointers
+// cast to the global address space (flat instructions otherwise).
 struct BatchMvDesc {
     const uint64_t *indices;
     const double2 *data;
@@ -92,15 +96,16 @@ struct BatchMvDesc {
 // (smaller items leave less of a tail).
 constexpr uint32_t kMvRows = 128;
 
-// Rows [r0, r1) of one operator's y = A x, 8 lanes per row; the CSR stream is
-// loaded nontemporally (kNT) for long rows only, as in k_csr_matvec.
-template <bool kNT>
+// Rows [r0, r1) of one operator's y = A x, 8 lanes per row (rows of fewer
+// than kMvStagedMinRow entries; plain loads, which beat nontemporal ones for
+// short rows as in k_csr_matvec).
 __device__ __forceinline__ void batch_mv_rows(const BatchMvDesc &d, uint64_t r0, uint64_t r1,
                                               uint32_t lane) {
-    auto ld = [](const auto *p) {
-        if constexpr (kNT) return __builtin_nontemporal_load(p);
-        else return *p;
-    };
+    auto ld = [](const auto p) { return *p; };
+    const auto indices = as_global(d.indices);
+    const auto data = as_global(reinterpret_cast<const dvec2 *>(d.data));
+    const auto x = as_global(reinterpret_cast<const dvec2 *>(d.x));
+    const auto y = as_global(reinterpret_cast<dvec2 *>(d.y));
     for (uint64_t rb = r0; rb < r1; rb += 32) {
         const uint64_t row = rb + (threadIdx.x >> 3);
         double2 acc = make_double2(0.0, 0.0);
@@ -113,13 +118,13 @@ __device__ __forceinline__ void batch_mv_rows(const BatchMvDesc &d, uint64_t r0,
                 double2 a[8];
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
-                    c[u] = ld(d.indices + k + 8 * u);
-                    const dvec2 av = ld(reinterpret_cast<const dvec2 *>(d.data + k + 8 * u));
+                    c[u] = ld(indices + k + 8 * u);
+                    const dvec2 av = ld(data + k + 8 * u);
                     a[u] = make_double2(av.x, av.y);
                 }
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
-                    const double2 xb = d.x[c[u]];
+                    const dvec2 xb = x[c[u]];
                     acc.x = fma(a[u].x, xb.x, acc.x);
                     acc.x = fma(-a[u].y, xb.y, acc.x);
                     acc.y = fma(a[u].x, xb.y, acc.y);
@@ -136,15 +141,15 @@ __device__ __forceinline__ void batch_mv_rows(const BatchMvDesc &d, uint64_t r0,
                     c[u] = 0;
                     a[u] = make_double2(0.0, 0.0);
                     if (k + 8 * u < k1) {
-                        c[u] = ld(d.indices + k + 8 * u);
-                        const dvec2 av = ld(reinterpret_cast<const dvec2 *>(d.data + k + 8 * u));
+                        c[u] = ld(indices + k + 8 * u);
+                        const dvec2 av = ld(data + k + 8 * u);
                         a[u] = make_double2(av.x, av.y);
                     }
                 }
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
                     if (k + 8 * u < k1) {
-                        const double2 xb = d.x[c[u]];
+                        const dvec2 xb = x[c[u]];
                         acc.x = fma(a[u].x, xb.x, acc.x);
                         acc.x = fma(-a[u].y, xb.y, acc.x);
                         acc.y = fma(a[u].x, xb.y, acc.y);
@@ -158,15 +163,24 @@ __device__ __forceinline__ void batch_mv_rows(const BatchMvDesc &d, uint64_t r0,
             acc.x += __shfl_xor(acc.x, off, 8);
             acc.y += __shfl_xor(acc.y, off, 8);
         }
-        if (row < r1 && lane == 0) d.y[row] = acc;
+        if (row < r1 && lane == 0) y[row] = dvec2{acc.x, acc.y};
     }
 }
 
-__global__ __launch_bounds__(256) void k_batch_mv(const BatchMvDesc *__restrict__ ops, uint32_t nops,
+// dynamic LDS: the row-staged walker's regions for the block's 4 waves
+constexpr size_t kBatchMvLds = 4 * MvStagedShape<kMvStagedChunk>::kWaveBytes;
+
+// (3 waves per SIMD: the LDS of three blocks fills a CU)
+__global__ __launch_bounds__(256, 3) void k_batch_mv(const BatchMvDesc *__restrict__ ops, uint32_t nops,
                                                   uint64_t total_items,
                                                   unsigned long long *__restrict__ counter) {
+    using S = MvStagedShape<kMvStagedChunk>;
+    extern __shared__ __align__(16) unsigned char mv_lds[];
     __shared__ unsigned long long item_s;
     const uint32_t lane = threadIdx.x & 7u;
+    const uint32_t wave = threadIdx.x / 64;
+    uint64_t *s_idx = reinterpret_cast<uint64_t *>(mv_lds + size_t(wave) * S::kWaveBytes);
+    dvec2 *s_dat = reinterpret_cast<dvec2 *>(s_idx + S::kRows * S::kIdxStride);
     for (;;) {
         if (threadIdx.x == 0) item_s = atomicAdd(counter, 1ull);
         __syncthreads();
@@ -182,8 +196,18 @@ __global__ __launch_bounds__(256) void k_batch_mv(const BatchMvDesc *__restrict_
         const BatchMvDesc &d = ops[lo];
         const uint64_t r0 = (item - d.item_begin) * kMvRows;
         const uint64_t r1 = r0 + kMvRows < d.rows ? r0 + kMvRows : d.rows;
-        if (d.G >= kMvNontemporalMinRow) batch_mv_rows<true>(d, r0, r1, lane);
-        else batch_mv_rows<false>(d, r0, r1, lane);
+        if (d.G >= kMvStagedMinRow) {
+            const uint64_t G = d.G;
+            auto bounds = [G](uint64_t r, uint64_t *k0, uint64_t *len) {
+                *k0 = r * G;
+                *len = G;
+            };
+            for (uint64_t rb = r0; rb < r1; rb += 4 * S::kRows)
+                mv_staged_rows<kMvStagedChunk>(rb + wave * S::kRows, r1, d.rows, bounds, d.indices,
+                                               d.data, d.x, d.y, s_idx, s_dat, 0, 0);
+        } else {
+            batch_mv_rows(d, r0, r1, lane);
+        }
     }
 }
 
@@ -362,7 +386,7 @@ int record(qr_batch *b, const Plan &plan, const qr_tuning &tu, uint64_t *const *
         // 8 resident 256-thread blocks per CU and slot
         const uint64_t grid = std::min<uint64_t>(uint64_t(b->ncu) * 8 * bpc, plan.mv_items);
         hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, s, b->d_counter + kClasses);
-        hipLaunchKernelGGL(k_batch_mv, dim3(unsigned(grid)), dim3(256), 0, s, b->d_mv,
+        hipLaunchKernelGGL(k_batch_mv, dim3(unsigned(grid)), dim3(256), kBatchMvLds, s, b->d_mv,
                            uint32_t(plan.mv.size()), plan.mv_items, b->d_counter + kClasses);
         QR_HIP_CHECK(hipGetLastError());
     }

@@ -24,6 +24,7 @@
 // non-temporal loads, gathers x, accumulates in FP64 with FMAs and reduces the
 // team with cross-lane shuffles.
 #include "qr_internal.h"
+#include "qr_mv.h"
 #include "qr_tile.h"
 
 #include <algorithm>
@@ -312,29 +313,8 @@ __global__ __launch_bounds__(256) void k_csr_matvec_short(uint64_t nrows, uint64
     }
 }
 
-// Long rows (at least kMvStagedMinRow entries on average): the row stream and
-// the x gathers want different lane layouts.  Streams run fastest when every
-// load instruction reads one contiguous run (measured on the 20-qubit,
-// 2000-term CSR with the gathers left out: 7.77 ms with a whole wave per row,
-// 9.23 ms with 8 lanes per row, where an instruction reads 8 rows in 64/128 B
-// pieces, half of them straddling two lines).  Gathers want the 8 lanes that
-// read one x line in the same instruction: for the Pauli CSR, rows that differ
-// in their low 3 bits read the same line at the same position of their
-// (column-sorted) rows, so a gather instruction should cover 8 rows at a few
-// positions each -- with a wave per row each gather instruction touches 64
-// lines and the matvec takes 16.2 ms.
-//
-// So each wave owns 8 consecutive rows and walks them in chunks of C
-// positions: it streams each row's chunk with contiguous loads (one row per
-// instruction for C = 64), writes it to its own padded LDS region, reads it
-// back transposed -- lane (s, t) takes positions p(s, u) of row t -- and
-// gathers x there.  The next chunk's stream loads are issued after the
-// gathers, so that waiting on a gather does not wait on them.  The paddings
-// keep both LDS access patterns free of bank conflicts.  Waves never
-// synchronise with each other (rows of different lengths run different
-// chunk counts).
-//
-// Which band of x the resident waves read.  In a column-sorted row, chunk c
+// k_csr_matvec_staged (the walker of qr_mv.h over a general CSR): which band
+// of x the resident waves read.  In a column-sorted row, chunk c
 // covers about the same band of x in every row (1/31 of x, 0.5 MB, for the
 // 2000-term CSR), so waves working on the same chunk index read the same
 // band, and the XCD's 4 MB L2 serves them.  Blocks start whenever a slot
@@ -349,137 +329,6 @@ __global__ __launch_bounds__(256) void k_csr_matvec_short(uint64_t nrows, uint64
 // resident set of blocks looping over the rows, all starting at chunk 0)
 // took 10.29 ms, its waves drifting apart.  The summation order depends only
 // on the block index, so results are reproducible.
-constexpr uint64_t kMvStagedMinRow = 256;
-constexpr int kMvStagedChunk = 64;
-
-template <int C>
-struct MvStagedShape {
-    static constexpr int kRows = 8;              // rows per wave
-    static constexpr int kRowsPerLoad = 64 / C;  // rows covered by one load instruction
-    static constexpr int kLoads = kRows / kRowsPerLoad;
-    static constexpr int kU = C / 8;             // positions per lane per chunk (gather layout)
-    // C = 64: lane (s, t) takes positions 8s..8s+7; C = 32: positions s, s+8, s+16, s+24
-    static constexpr bool kRun = C == 64;
-    static constexpr int kIdxStride = C == 64 ? C + 1 : C + 2;   // u64 slots per row
-    static constexpr int kDatStride = C + 1;                     // double2 slots per row
-    static constexpr size_t kWaveBytes = size_t(kRows) * (kIdxStride * 8 + kDatStride * 16);
-    __device__ static constexpr int pos(int s, int u) { return kRun ? s * kU + u : u * 8 + s; }
-};
-
-__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int lane) {
-    const uint32_t lo = __builtin_amdgcn_readlane(uint32_t(v), lane);
-    const uint32_t hi = __builtin_amdgcn_readlane(uint32_t(v >> 32), lane);
-    return (uint64_t(hi) << 32) | lo;
-}
-
-// One wave's 8 rows r0..r0+7 (rows >= nrows are empty).
-template <int C>
-__device__ __forceinline__ void mv_staged_rows(uint64_t r0, uint64_t nrows, uint64_t ncols, uint64_t nnz,
-                                               const uint64_t *__restrict__ indptr,
-                                               const uint64_t *__restrict__ indices,
-                                               const double2 *__restrict__ data,
-                                               const double2 *__restrict__ x,
-                                               double2 *__restrict__ y, uint64_t *s_idx, dvec2 *s_dat,
-                                               uint64_t i_blk, uint64_t k_res) {
-    using S = MvStagedShape<C>;
-    constexpr int R = S::kRows;
-    const int l = int(threadIdx.x % 64);
-    const int s = l / 8, t = l % 8;
-
-    // row bounds, clamped to [0, nnz): lane j < 8 holds row r0 + j
-    uint64_t k0l = 0, lenl = 0;
-    if (l < R && r0 + l < nrows) {
-        const uint64_t a = min(indptr[r0 + l], nnz), e = min(indptr[r0 + l + 1], nnz);
-        k0l = a;
-        lenl = e > a ? e - a : 0;
-    }
-    uint64_t k0[R], len[R], maxlen = 0;
-#pragma unroll
-    for (int j = 0; j < R; ++j) {
-        k0[j] = readlane_u64(k0l, j);
-        len[j] = readlane_u64(lenl, j);
-        maxlen = max(maxlen, len[j]);
-    }
-    const uint64_t lent = __shfl(lenl, t);  // length of this lane's gather row
-    const uint64_t nchunks = (maxlen + C - 1) / C;
-
-    // stream registers: load instruction q covers row q * kRowsPerLoad + l / C
-    uint64_t si[S::kLoads];
-    dvec2 sd[S::kLoads];
-    auto stream = [&](uint64_t c) {
-#pragma unroll
-        for (int q = 0; q < S::kLoads; ++q) {
-            const uint64_t kb = S::kRowsPerLoad == 1 ? k0[q] : (l < C ? k0[2 * q] : k0[2 * q + 1]);
-            const uint64_t lj = S::kRowsPerLoad == 1 ? len[q] : (l < C ? len[2 * q] : len[2 * q + 1]);
-            const uint64_t p = c * C + uint64_t(l % C);
-            if (p < lj) {
-                si[q] = __builtin_nontemporal_load(indices + kb + p);
-                sd[q] = __builtin_nontemporal_load(reinterpret_cast<const dvec2 *>(data + kb + p));
-            }
-        }
-    };
-#pragma unroll
-    for (int q = 0; q < S::kLoads; ++q) {
-        si[q] = ~uint64_t(0);
-        sd[q] = dvec2{0.0, 0.0};
-    }
-    // first chunk: this block's share of the resident blocks' stagger
-    const uint64_t c0 = nchunks ? (i_blk % k_res) * nchunks / k_res : 0;
-    if (nchunks) stream(c0);
-
-    double2 acc = make_double2(0.0, 0.0);
-    for (uint64_t cc = 0, c = c0; cc < nchunks; ++cc) {
-#pragma unroll
-        for (int q = 0; q < S::kLoads; ++q) {
-            const int j = q * S::kRowsPerLoad + (S::kRowsPerLoad == 1 ? 0 : l / C);
-            s_idx[j * S::kIdxStride + l % C] = si[q];
-            s_dat[j * S::kDatStride + l % C] = sd[q];
-        }
-        // LDS writes of other lanes visible to this wave's reads below
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        uint64_t gi[S::kU];
-        dvec2 gd[S::kU];
-        double2 gx[S::kU];
-        bool ok[S::kU];
-#pragma unroll
-        for (int u = 0; u < S::kU; ++u) {
-            const int p = S::pos(s, u);
-            gi[u] = s_idx[t * S::kIdxStride + p];
-            gd[u] = s_dat[t * S::kDatStride + p];
-            ok[u] = (c * C + uint64_t(p) < lent) && gi[u] < ncols;
-        }
-#pragma unroll
-        for (int u = 0; u < S::kU; ++u) gx[u] = ok[u] ? x[gi[u]] : make_double2(0.0, 0.0);
-        // the next chunk's stream loads go out behind the gathers
-        __builtin_amdgcn_sched_barrier(0);
-        const uint64_t cn = c + 1 == nchunks ? 0 : c + 1;
-        if (cc + 1 < nchunks) stream(cn);
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int u = 0; u < S::kU; ++u) {
-            if (ok[u]) {
-                acc.x = fma(gd[u].x, gx[u].x, acc.x);
-                acc.x = fma(-gd[u].y, gx[u].y, acc.x);
-                acc.y = fma(gd[u].x, gx[u].y, acc.y);
-                acc.y = fma(gd[u].y, gx[u].x, acc.y);
-            }
-        }
-        // this chunk's LDS reads done before the next chunk's writes
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        c = cn;
-    }
-#pragma unroll
-    for (int off = 8; off < 64; off *= 2) {
-        acc.x += __shfl_xor(acc.x, off);
-        acc.y += __shfl_xor(acc.y, off);
-    }
-    if (s == 0 && r0 + t < nrows) y[r0 + t] = acc;
-}
-
 // One block per 32 rows; block index remapped so that each XCD works on one
 // contiguous range of rows (blocks go to the 8 XCDs round-robin).
 template <int C>
@@ -497,8 +346,15 @@ __global__ __launch_bounds__(256) void k_csr_matvec_staged(uint64_t nrows, uint6
     const int wave = int(threadIdx.x / 64);
     uint64_t *s_idx = reinterpret_cast<uint64_t *>(mv_lds + size_t(wave) * S::kWaveBytes);
     dvec2 *s_dat = reinterpret_cast<dvec2 *>(s_idx + S::kRows * S::kIdxStride);
-    mv_staged_rows<C>((uint64_t(b) * 4 + uint64_t(wave)) * S::kRows, nrows, ncols, nnz, indptr,
-                      indices, data, x, y, s_idx, s_dat, blockIdx.x / 8, k_res);
+    // row bounds clamped to [0, nnz), so that a malformed indptr cannot read out
+    // of bounds
+    auto bounds = [indptr, nnz](uint64_t r, uint64_t *k0, uint64_t *len) {
+        const uint64_t a = min(indptr[r], nnz), e = min(indptr[r + 1], nnz);
+        *k0 = a;
+        *len = e > a ? e - a : 0;
+    };
+    mv_staged_rows<C>((uint64_t(b) * 4 + uint64_t(wave)) * S::kRows, nrows, ncols, bounds, indices,
+                      data, x, y, s_idx, s_dat, blockIdx.x / 8, k_res);
 }
 
 // Blocks of k_csr_matvec_staged resident at once on the current device (per

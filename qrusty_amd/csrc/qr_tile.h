@@ -21,12 +21,11 @@
 #pragma once
 
 #include "qr_internal.h"
+#include "qr_mv.h"
 
 #include <type_traits>
 
 namespace qr {
-
-typedef double dvec2 __attribute__((ext_vector_type(2)));
 
 constexpr int kBuildM = 16;            // batches per tile (4 varying row bits)
 constexpr int kStageThreads = 1024;
@@ -167,15 +166,14 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
     const uint64_t rfix = tile_base | ro;
     const uint32_t batch = G << rs_log2;  // entries per batch of RS rows
     const uint32_t batch_pad = (batch + 1u) & ~1u;
-    uint64_t *st_idx[2];
-    double2 *st_dat[2];
-    st_idx[0] = reinterpret_cast<uint64_t *>(smem_raw);
-    st_dat[0] = reinterpret_cast<double2 *>(st_idx[0] + batch_pad);
-    st_idx[1] = reinterpret_cast<uint64_t *>(st_dat[0] + batch_pad);
-    st_dat[1] = reinterpret_cast<double2 *>(st_idx[1] + batch_pad);
-    double2 *red = reinterpret_cast<double2 *>(op.dbuf ? reinterpret_cast<unsigned char *>(st_idx[1]) +
-                                                             size_t(batch_pad) * 24
-                                                       : reinterpret_cast<unsigned char *>(st_idx[1]));
+    // stage b (index + value of one batch) at byte b * stage_bytes; the
+    // pointers are formed from smem_raw and a byte offset (not picked from an
+    // array of pointers), so that the compiler keeps them in the LDS address
+    // space: through generic pointers the stage accesses become flat
+    // instructions, which count against vmcnt, and every copy-out load then
+    // waited for all earlier global stores
+    const uint32_t stage_b = batch_pad * 24u;
+    double2 *red = reinterpret_cast<double2 *>(smem_raw + (op.dbuf ? 2u : 1u) * stage_b);
     // heavy-group values of the tile's rows: hv[h << tile_log2 | (r - tile_base)]
     double2 *hv = reinterpret_cast<double2 *>(smem_raw + stage_bytes(G, jw_log2, nullptr));
 
@@ -284,8 +282,8 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
 #pragma unroll 1
     for (int m = 0; m < kBuildM; ++m) {
         const int buf = op.dbuf ? (m & 1) : 0;
-        uint64_t *si = st_idx[buf];
-        double2 *sd = st_dat[buf];
+        uint64_t *si = reinterpret_cast<uint64_t *>(smem_raw + uint32_t(buf) * stage_b);
+        double2 *sd = reinterpret_cast<double2 *>(si + batch_pad);
         const uint64_t r = rfix + (uint64_t(m) << rs_log2);
         const bool row_ok = r >= row_begin && r < row_end;
         const int32_t off = kSub ? off_fix + ((m & 1) ? o0 : 0) + ((m & 2) ? o1 : 0) +
