@@ -132,7 +132,9 @@ def bench_csr(args, torch, qa):
     }
     res["build_pct_hbm"] = 100 * res["build_GBps"] / HBM_PEAK_GBPS
     res["matvec_pct_hbm"] = 100 * res["matvec_GBps"] / HBM_PEAK_GBPS
-    dom = ("k_csr_build", bm, b_build) if bm >= mm else ("k_csr_matvec", mm, b_mv)
+    # the kernels that run here: k_csr_build_staged (G <= 2048) and
+    # k_csr_matvec_staged (rows of 256+ entries)
+    dom = ("k_csr_build_staged", bm, b_build) if bm >= mm else ("k_csr_matvec_staged", mm, b_mv)
     res["roofline"] = {
         "kernel": dom[0], "bound": "hbm", "achieved": dom[2] / dom[1] / 1e6, "peak": HBM_PEAK_GBPS,
         "unit": "GB/s", "frac": dom[2] / dom[1] / 1e6 / HBM_PEAK_GBPS,

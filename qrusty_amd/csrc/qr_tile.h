@@ -76,6 +76,52 @@ struct TileOp {
     uint32_t gbase, nanc, gstride;
 };
 
+// The arrays of a TileOp seen through pointers typed as global memory.  A
+// TileOp that reaches the kernel inside a descriptor in memory (the batch)
+// holds generic pointers, which the compiler would access with flat
+// instructions: 64-bit per-lane addresses, counted against both vmcnt and
+// lgkmcnt.  The builder reads and writes only through this view.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T *;
+struct TileMem {
+    gptr<const uint64_t> gx;
+    gptr<const uint32_t> gterm;
+    gptr<const uint64_t> tz;
+    gptr<const dvec2> tc;
+    gptr<const uint32_t> rl_ptr;
+    gptr<const RankEntry> rl;
+    gptr<uint64_t> indices;
+    gptr<dvec2> data;
+    gptr<uint64_t> indptr;
+    gptr<const dvec2> x;
+    gptr<dvec2> y;
+    gptr<const uint32_t> heavy;
+};
+__device__ __forceinline__ TileMem tile_mem(const TileOp &op) {
+    return {as_global(op.gx),
+            as_global(op.gterm),
+            as_global(op.tz),
+            as_global(reinterpret_cast<const dvec2 *>(op.tc)),
+            as_global(op.rl_ptr),
+            as_global(op.rl),
+            as_global(op.indices),
+            as_global(reinterpret_cast<dvec2 *>(op.data)),
+            as_global(op.indptr),
+            as_global(reinterpret_cast<const dvec2 *>(op.x)),
+            as_global(reinterpret_cast<dvec2 *>(op.y)),
+            as_global(op.heavy)};
+}
+__device__ __forceinline__ RankEntry ld_rank(gptr<const RankEntry> p) {
+    RankEntry e;
+    e.bit = p->bit;
+    e.sigma = p->sigma;
+    return e;
+}
+__device__ __forceinline__ double2 ld2(gptr<const dvec2> p) {
+    const dvec2 v = *p;
+    return make_double2(v.x, v.y);
+}
+
 // LDS bytes for one operator's stage: (one or two) x (index + value) x batch,
 // plus 2 x 16 wave partials for the fused matvec.  The stage can be
 // double-buffered (the copy-out of batch m overlaps the build of m+1) when both
@@ -111,15 +157,15 @@ __host__ __device__ inline uint32_t heavy_fit(uint32_t nheavy, uint32_t jw_log2)
 // (eight at a time cost the heavy-group instance its second block per CU).
 // Measured on the 24-qubit Heisenberg build: 2.68 -> 2.58 ms.
 template <bool kNarrow>
-__device__ __forceinline__ double2 fold_group(const TileOp &op, uint32_t j, uint64_t r) {
-    const uint32_t t0 = op.gterm[j], t1 = op.gterm[j + 1];
-    double2 v = signed_coeff(op.tc[t0], parity_and<kNarrow>(r, op.tz[t0]));
+__device__ __forceinline__ double2 fold_group(const TileMem &mem, uint32_t j, uint64_t r) {
+    const uint32_t t0 = mem.gterm[j], t1 = mem.gterm[j + 1];
+    double2 v = signed_coeff(ld2(mem.tc + t0), parity_and<kNarrow>(r, mem.tz[t0]));
     uint32_t t = t0 + 1;
     constexpr uint32_t kF = 4;
     for (; t + kF <= t1; t += kF) {
         double2 ct[kF];
 #pragma unroll
-        for (uint32_t u = 0; u < kF; ++u) ct[u] = signed_coeff(op.tc[t + u], parity_and<kNarrow>(r, op.tz[t + u]));
+        for (uint32_t u = 0; u < kF; ++u) ct[u] = signed_coeff(ld2(mem.tc + t + u), parity_and<kNarrow>(r, mem.tz[t + u]));
 #pragma unroll
         for (uint32_t u = 0; u < kF; ++u) {
             v.x = v.x + ct[u].x;
@@ -127,7 +173,7 @@ __device__ __forceinline__ double2 fold_group(const TileOp &op, uint32_t j, uint
         }
     }
     for (; t < t1; ++t) {
-        const double2 ct = signed_coeff(op.tc[t], parity_and<kNarrow>(r, op.tz[t]));
+        const double2 ct = signed_coeff(ld2(mem.tc + t), parity_and<kNarrow>(r, mem.tz[t]));
         v.x = v.x + ct.x;
         v.y = v.y + ct.y;
     }
@@ -155,6 +201,7 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
     // room; with more than kStageMaxChunks not even the first one does
     constexpr bool kHold2 = kChunks <= 2;
     constexpr bool kLean = kChunks > kStageMaxChunks;
+    const TileMem mem = tile_mem(op);
     const uint32_t G = op.G, jw_log2 = op.jw_log2, nchunks = op.nchunks;
     const uint64_t row_begin = op.row_begin, row_end = op.row_end;
     const uint32_t rs_log2 = 10u - jw_log2;
@@ -183,8 +230,8 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
         const uint64_t stride = kSub ? op.gstride : G;
         for (uint64_t i = tid; i < tr; i += kStageThreads) {
             const uint64_t r = tile_base + i;
-            if (r >= row_begin && r < row_end) op.indptr[r - row_begin] = r * stride;
-            if (r + 1 == row_end) op.indptr[r + 1 - row_begin] = (r + 1) * stride;
+            if (r >= row_begin && r < row_end) mem.indptr[r - row_begin] = r * stride;
+            if (r + 1 == row_end) mem.indptr[r + 1 - row_begin] = (r + 1) * stride;
         }
     }
     // a sub-operator's first rank in the row: fixed bits summed, batch bits
@@ -194,7 +241,7 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
     int32_t off_fix = 0, o0 = 0, o1 = 0, o2 = 0, o3 = 0;
     if (kSub) off_fix = int32_t(op.gbase);
     for (uint32_t e = 0; kSub && e < op.nanc; ++e) {
-        const RankEntry en = op.rl[op.rl_ptr[0] + e];
+        const RankEntry en = ld_rank(mem.rl + mem.rl_ptr[0] + e);
         const uint32_t b = uint32_t(en.bit);
         const uint32_t q = b - rs_log2;  // wraps for b < rs_log2
         if (q == 0) o0 = en.sigma;
@@ -230,10 +277,10 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
             uint32_t lo = 0, hi = op.nheavy;
             while (lo < hi) {
                 const uint32_t mid = (lo + hi) >> 1;
-                if (op.heavy[mid] < j) lo = mid + 1;
+                if (mem.heavy[mid] < j) lo = mid + 1;
                 else hi = mid;
             }
-            if (lo < op.nheavy && op.heavy[lo] == j) hs[c] = int32_t(lo);
+            if (lo < op.nheavy && mem.heavy[lo] == j) hs[c] = int32_t(lo);
         }
         int32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
         s01[c] = s23[c] = 0;
@@ -241,9 +288,9 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
         c0[c] = c1[c] = make_double2(0.0, 0.0);
         t0[c] = t1[c] = 0;
         if (uint32_t(c) >= nchunks || j >= G) continue;
-        const uint32_t e1 = op.rl_ptr[j + 1];
-        for (uint32_t e = op.rl_ptr[j]; e < e1; ++e) {
-            const RankEntry en = op.rl[e];
+        const uint32_t e1 = mem.rl_ptr[j + 1];
+        for (uint32_t e = mem.rl_ptr[j]; e < e1; ++e) {
+            const RankEntry en = ld_rank(mem.rl + e);
             const uint32_t b = uint32_t(en.bit);
             const uint32_t q = b - rs_log2;  // wraps for b < rs_log2
             if (q == 0) s0 = en.sigma;
@@ -254,16 +301,16 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
         }
         s01[c] = int32_t((uint32_t(s1) << 16) | (uint32_t(s0) & 0xffffu));
         s23[c] = int32_t((uint32_t(s3) << 16) | (uint32_t(s2) & 0xffffu));
-        xj[c] = mask_t(op.gx[j]);
-        t0[c] = op.gterm[j];
-        t1[c] = op.gterm[j + 1];
+        xj[c] = mask_t(mem.gx[j]);
+        t0[c] = mem.gterm[j];
+        t1[c] = mem.gterm[j + 1];
         if (!kLean) {
-            z0[c] = mask_t(op.tz[t0[c]]);
-            c0[c] = op.tc[t0[c]];
+            z0[c] = mask_t(mem.tz[t0[c]]);
+            c0[c] = ld2(mem.tc + t0[c]);
         }
         if (kHold2 && t1[c] > t0[c] + 1) {
-            z1[c] = mask_t(op.tz[t0[c] + 1]);
-            c1[c] = op.tc[t0[c] + 1];
+            z1[c] = mask_t(mem.tz[t0[c] + 1]);
+            c1[c] = ld2(mem.tc + t0[c] + 1);
         }
     }
 
@@ -274,7 +321,7 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
         for (uint32_t i = tid; i < op.nheavy * tr; i += kStageThreads) {
             const uint64_t r = tile_base + (i & (tr - 1));
             if (r >= row_begin && r < row_end)
-                hv[i] = fold_group<kNarrow>(op, op.heavy[i / tr], r);
+                hv[i] = fold_group<kNarrow>(mem, mem.heavy[i / tr], r);
         }
         __syncthreads();
     }
@@ -307,25 +354,25 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
             uint32_t tf = t0[c];
             asm volatile("" : "+v"(tf));
             // term-order fold: first value, then + (accel.rs:191-208)
-            double2 v = kLean ? signed_coeff(op.tc[tf], parity_and<kNarrow>(r, op.tz[tf]))
+            double2 v = kLean ? signed_coeff(ld2(mem.tc + tf), parity_and<kNarrow>(r, mem.tz[tf]))
                               : signed_coeff(c0[c], parity_and<kNarrow>(r, z0[c]));
             if (kHeavy && hs[c] >= 0) {
                 v = hv[(uint32_t(hs[c]) << tile_log2) | (uint32_t(m) << rs_log2) | ro];
             } else if (t1[c] > tf + 1) {
                 const double2 e = kHold2 ? signed_coeff(c1[c], parity_and<kNarrow>(r, z1[c]))
-                                         : signed_coeff(op.tc[tf + 1],
-                                                        parity_and<kNarrow>(r, op.tz[tf + 1]));
+                                         : signed_coeff(ld2(mem.tc + tf + 1),
+                                                        parity_and<kNarrow>(r, mem.tz[tf + 1]));
                 v.x = v.x + e.x;
                 v.y = v.y + e.y;
                 for (uint32_t t = tf + 2; t < t1[c]; ++t) {
-                    const double2 ct = signed_coeff(op.tc[t], parity_and<kNarrow>(r, op.tz[t]));
+                    const double2 ct = signed_coeff(ld2(mem.tc + t), parity_and<kNarrow>(r, mem.tz[t]));
                     v.x = v.x + ct.x;
                     v.y = v.y + ct.y;
                 }
             }
             const uint64_t col = r ^ uint64_t(xj[c]);
             if (kMatvec) {
-                const double2 xb = op.x[col];
+                const double2 xb = ld2(mem.x + col);
                 acc.x = fma(v.x, xb.x, acc.x);
                 acc.x = fma(-v.y, xb.y, acc.x);
                 acc.y = fma(v.x, xb.y, acc.y);
@@ -351,7 +398,7 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
                 acc.y += __shfl_xor(acc.y, int(off), 64);
             }
             if (jw_log2 <= 6) {
-                if (lane == 0 && rr >= row_begin && rr < row_end) op.y[rr - row_begin] = acc;
+                if (lane == 0 && rr >= row_begin && rr < row_end) mem.y[rr - row_begin] = dvec2{acc.x, acc.y};
             } else if ((tid & 63) == 0) {
                 red[buf * 16 + (tid >> 6)] = acc;
             }
@@ -364,10 +411,10 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
             const uint64_t dst = kSub ? (lo_r - row_begin) * op.gstride + uint32_t(off)
                                       : (lo_r - row_begin) * G;
             for (uint32_t i = tid; i < count; i += kStageThreads) {
-                op.indices[dst + i] = si[src + i];
+                mem.indices[dst + i] = si[src + i];
                 const double2 dv = sd[src + i];
                 const dvec2 vv = {dv.x, dv.y};
-                *reinterpret_cast<dvec2 *>(op.data + dst + i) = vv;
+                mem.data[dst + i] = vv;
             }
         }
         if (kMatvec && jw_log2 > 6) {
@@ -382,7 +429,7 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
                     sum.y += p.y;
                 }
                 const uint64_t rr = b0 + tid;
-                if (rr >= row_begin && rr < row_end) op.y[rr - row_begin] = sum;
+                if (rr >= row_begin && rr < row_end) mem.y[rr - row_begin] = dvec2{sum.x, sum.y};
             }
         } else if (!op.dbuf) {
             __syncthreads();
