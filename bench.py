@@ -369,13 +369,12 @@ def bench_batch(args, torch, qa):
     ms = [ev[k].elapsed_time(ev[k + 1]) for k in range(args.steps)]
     m = statistics.mean(ms)
     # algorithmic bytes: CSR written (24 B/nnz + indptr), then y = A x (x read,
-    # y written).  The split form's matvec pass reads every CSR back; the
-    # fused form reads back only the operators of more than 6144 groups, which
-    # run the single-operator kernels.  In the split form every operator of
-    # more than 2048 groups is built by the single-operator (sub-operator)
-    # kernels.
+    # y written).  The split form's matvec pass reads every CSR back (its two
+    # build kernels take every operator as sub-operators); the fused form
+    # reads back only the operators of more than 6144 groups, which run the
+    # single-operator kernels.
     split = bool(best.get("split", 0))
-    big = [k for k, h in enumerate(b._h) if h.G > (2048 if split else 6144)]
+    big = [] if split else [k for k, h in enumerate(b._h) if h.G > 6144]
     nnz_big = sum(b._h[k].nnz for k in big)
     nnz_reread = nnz if split else sum(h.nnz for h in b._h if h.G > 6144)
     byts = 24 * nnz + 8 * (rows + len(ops)) + 24 * nnz_reread + 32 * rows

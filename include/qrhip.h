@@ -334,14 +334,20 @@ int qr_pauli_apply_noalloc(const qr_op *op, int32_t precision, uint32_t nvec, co
 /* ------------------------------------------------------------------------
  * Batches (no counterpart in the reference, which builds and applies one
  * operator at a time: lib.rs:573-585, accel.rs:338-370).  One call builds the
- * CSR of every operator of the batch and one matvec y_k = A_k x_k each, in a
- * single persistent launch recorded in a hipGraph (replayed while the buffers
- * and tuning stay the same).  CSR entries are bit-identical to qr_csr_build;
- * y_k is computed from the freshly built entries (FP64 FMAs).
+ * CSR of every operator of the batch and one matvec y_k = A_k x_k each, as
+ * persistent kernels that take (operator, row tile) items from atomic
+ * counters, recorded in a hipGraph (replayed while the buffers and tuning stay
+ * the same).  Split form: two build kernels, whose items are every
+ * operator's tiles (as sub-operators of at most 2048 groups), then one matvec
+ * kernel over the written CSRs.  Fused form: one build-and-apply kernel per
+ * register class, plus the single-operator kernels for operators of more
+ * than 6144 groups.  CSR entries are bit-identical to qr_csr_build; y_k is
+ * computed in FP64 with FMAs.
  *   d_indptr[k], d_indices[k], d_data[k] : CSR outputs sized as for
  *       qr_csr_build, or NULL (all three, or the arrays themselves) to keep
- *       only y_k -- except for operators with more than 6144 distinct x masks,
- *       which run through the single-operator kernels and need them
+ *       only y_k (fused form) -- except for operators with more than 6144
+ *       distinct x masks, which need them; the split form needs d_indices
+ *       and d_data of every operator, d_indptr is optional
  *   d_x[k], d_y[k] : complex128 states of 2^n_k entries (n_k <= 32)
  *   tuning : NULL = the tuned (or default) parameters
  * qr_batch_tune times every tuning candidate on the given buffers

@@ -9,10 +9,14 @@
 // bit-identical to the single-operator build -- and, fused, the same rows of
 // y = A x from the entries still in registers.  No item depends on another,
 // so every block leaves the loop as soon as the counter passes the end.
-// Operators whose G exceeds the largest LDS stage (kStageMaxGLean) are built
-// and applied by the single-operator kernels, recorded in the same graph; in
-// the split form so are those with G > kSubMaxG (built as sub-operators of at
-// most kSubMaxG groups, two blocks per CU, then applied by k_batch_mv).
+// Two forms (a tuning choice).  Fused: one kernel per register class builds
+// and applies; operators whose G exceeds the largest LDS stage
+// (kStageMaxGLean) are built and applied by the single-operator kernels,
+// recorded in the same graph.  Split (needs every CSR): two build kernels,
+// whose items are the tiles of every operator as sub-operators of at most
+// kSubMaxG groups (an operator of at most kSubMaxG groups is one), then one
+// matvec pass (k_batch_mv) over the written CSRs -- three compute kernels per
+// step whatever the batch holds.
 //
 // Tuning knobs (qr_tuning): tiles per item (work granularity), resident
 // blocks per CU (grid size), and the item order (input order or largest
@@ -43,10 +47,11 @@ struct BatchOpDesc {
 // item loop costs a few more than the single-operator kernel), and without
 // the bound the 256-operator batch measured 92.0-93.1 against 93.0-94.1 ms.
 #ifndef QR_BATCH_WAVES
-#define QR_BATCH_WAVES(matvec, chunks) ((!(matvec) && (chunks) <= 1) ? 8 : 1)
+#define QR_BATCH_WAVES(matvec, chunks, mode) \
+    ((!(matvec) && ((chunks) <= 1 || (mode) == kTileSub)) ? 8 : 1)
 #endif
-template <bool kNarrow, int kChunks, bool kMatvec>
-__global__ __launch_bounds__(kStageThreads, QR_BATCH_WAVES(kMatvec, kChunks)) void k_batch(const BatchOpDesc *__restrict__ ops,
+template <bool kNarrow, int kChunks, bool kMatvec, int kMode = kTilePlain>
+__global__ __launch_bounds__(kStageThreads, QR_BATCH_WAVES(kMatvec, kChunks, kMode)) void k_batch(const BatchOpDesc *__restrict__ ops,
                                                          uint32_t nops, uint64_t total_items,
                                                          uint32_t tiles_per_item,
                                                          unsigned long long *__restrict__ counter) {
@@ -69,7 +74,7 @@ __global__ __launch_bounds__(kStageThreads, QR_BATCH_WAVES(kMatvec, kChunks)) vo
         const uint64_t t0 = (item - d.item_begin) * tiles_per_item;
         const uint64_t t1 = t0 + tiles_per_item < d.tiles ? t0 + tiles_per_item : d.tiles;
         for (uint64_t t = t0; t < t1; ++t) {
-            build_tile_staged<kNarrow, kChunks, kMatvec>(d.op, t, smem_raw);
+            build_tile_staged<kNarrow, kChunks, kMatvec, kMode>(d.op, t, smem_raw);
             __syncthreads();  // the stage is reused by the next tile
         }
     }
@@ -240,9 +245,12 @@ struct qr_batch {
 namespace {
 
 // register classes: group chunks per thread in {1, 2, kStageMaxChunks,
-// kStageLeanChunks}.  A batch operator builds all 2^n rows, so n <= 32 and
-// masks are 32-bit.
-constexpr int kClasses = 4;
+// kStageLeanChunks} for whole operators, and {1, 2} for the sub-operators
+// (classes 4 and 5) of the split form's operators of more than kSubMaxG
+// groups.  A batch operator builds all 2^n rows, so n <= 32 and masks are
+// 32-bit.
+constexpr int kClasses = 6;
+constexpr int kSubClass = 4;
 inline int class_of(uint32_t nch) {
     return nch <= 1 ? 0 : nch <= 2 ? 1 : nch <= uint32_t(kStageMaxChunks) ? 2 : 3;
 }
@@ -275,6 +283,17 @@ int make_plan(qr_batch *b, uint64_t *const *d_indptr, uint64_t *const *d_indices
         if (!(d_indices && d_indices[k] && d_data && d_data[k]))
             return fail(QR_ERR_ARG, "the split batch form needs CSR output buffers for every operator");
     std::vector<BatchOpDesc> per[kClasses];
+    auto add_item = [&](int c, const TileOp &top, uint64_t tiles, size_t shmem) {
+        BatchOpDesc d{};
+        d.op = top;
+        d.tiles = tiles;
+        d.narrow = 1;
+        d.nch = top.nchunks;
+        d.item_begin = plan.cls_items[c];
+        plan.cls_items[c] += (tiles + tpi - 1) / tpi;
+        plan.cls_shmem[c] = std::max(plan.cls_shmem[c], shmem);
+        per[c].push_back(d);
+    };
     for (uint32_t k : order) {
         const qr_op *op = b->ops[k];
         if (op->nq > 32) return fail(QR_ERR_SIZE, "batch operators are limited to 32 qubits");
@@ -285,48 +304,58 @@ int make_plan(qr_batch *b, uint64_t *const *d_indptr, uint64_t *const *d_indices
         if (rc) return rc;
         const uint32_t G = uint32_t(op->G);
         const uint64_t N = uint64_t(1) << op->nq;
-        if (op->G > kStageMaxGLean || (plan.split && !op->leaves.empty())) {
-            // built by the single-operator kernels (sub-operators of at most
-            // kSubMaxG groups); in the split form applied by k_batch_mv
-            if (!has_csr || !d_indptr || !d_indptr[k])
+        uint64_t *ip = d_indptr ? d_indptr[k] : nullptr;
+        auto *dat = has_csr ? reinterpret_cast<double2 *>(d_data[k]) : nullptr;
+        auto *xk = reinterpret_cast<const double2 *>(d_x[k]);
+        auto *yk = reinterpret_cast<double2 *>(d_y[k]);
+        if (plan.split) {
+            // split form: every operator in the two sub-operator classes (1 or
+            // 2 group chunks per thread).  One of at most kSubMaxG groups is a
+            // sub-operator of itself (gbase = nanc = 0, gstride = G); a larger
+            // one is its subtrees of the mask trie, each filling a contiguous
+            // segment of every row, the first of which writes indptr.  Heavy
+            // groups are folded inline.  The matvec pass then reads the CSR.
+            if (op->leaves.empty()) {
+                const uint32_t jw = stage_jw_log2(G), nch = (G + (1u << jw) - 1) >> jw;
+                uint32_t dbuf = 0;
+                const size_t sb = stage_bytes(G, jw, &dbuf);
+                add_item(kSubClass + (nch <= 1 ? 0 : 1),
+                         TileOp{t->gx, t->gterm, t->tz, t->tc, t->rl_ptr, t->rl, d_indices[k], dat, ip, xk,
+                                yk, 0, N, G, jw, nch, dbuf, nullptr, 0, 0, 0, G},
+                         ((N - 1) >> (10 - jw + 4)) + 1, sb);
+            } else {
+                for (const auto &lf : op->leaves) {
+                    const uint32_t g = lf.s1 - lf.s0, nch = (g + 1023) >> 10;
+                    uint32_t dbuf = 0;
+                    const size_t sb = stage_bytes(g, 10, &dbuf);
+                    add_item(kSubClass + (nch <= 1 ? 0 : 1),
+                             TileOp{t->gx + lf.s0, t->gterm + lf.s0, t->tz, t->tc, t->rl_ptr + lf.s0, t->rl,
+                                    d_indices[k], dat, lf.s0 == 0 ? ip : nullptr, xk, yk, 0, N, g, 10, nch,
+                                    dbuf, nullptr, 0, lf.s0, lf.depth, G},
+                             ((N - 1) >> 4) + 1, sb);
+                }
+            }
+            plan.mv.push_back(BatchMvDesc{d_indices[k], dat, xk, yk, N, G, plan.mv_items});
+            plan.mv_items += (N + kMvRows - 1) / kMvRows;
+            continue;
+        }
+        if (op->G > kStageMaxGLean) {
+            // fused form, more than kStageMaxGLean groups: built and applied
+            // by the single-operator kernels
+            if (!has_csr || !ip)
                 return fail(QR_ERR_ARG, "batch operators with more than 6144 distinct x masks "
                                         "need CSR output buffers");
             plan.big.push_back(k);
-            if (plan.split) {
-                BatchMvDesc m{d_indices[k], reinterpret_cast<const double2 *>(d_data[k]),
-                              reinterpret_cast<const double2 *>(d_x[k]),
-                              reinterpret_cast<double2 *>(d_y[k]), N, G, plan.mv_items};
-                plan.mv_items += (N + kMvRows - 1) / kMvRows;
-                plan.mv.push_back(m);
-            }
             continue;
         }
-        const uint32_t jw = stage_jw_log2(G);
+        // fused form: the whole operator in the class of its chunk count
+        const uint32_t jw = stage_jw_log2(G), nch = (G + (1u << jw) - 1) >> jw;
         uint32_t dbuf = 0;
         const size_t sb = stage_bytes(G, jw, &dbuf);
-        const uint32_t tl2 = 10 - jw + 4;
-        BatchOpDesc d{};
-        d.op = TileOp{t->gx, t->gterm, t->tz, t->tc, t->rl_ptr, t->rl,
-                      has_csr ? d_indices[k] : nullptr,
-                      has_csr ? reinterpret_cast<double2 *>(d_data[k]) : nullptr,
-                      d_indptr ? d_indptr[k] : nullptr,
-                      reinterpret_cast<const double2 *>(d_x[k]), reinterpret_cast<double2 *>(d_y[k]),
-                      0, N, G, jw, (G + (1u << jw) - 1) >> jw, dbuf, nullptr, 0, 0, 0, G};
-        d.tiles = ((N - 1) >> tl2) + 1;
-        d.narrow = 1;
-        d.nch = d.op.nchunks;
-        const int c = class_of(d.nch);
-        d.item_begin = plan.cls_items[c];
-        plan.cls_items[c] += (d.tiles + tpi - 1) / tpi;
-        plan.cls_shmem[c] = std::max(plan.cls_shmem[c], sb);
-        per[c].push_back(d);
-        if (plan.split) {
-            BatchMvDesc m{d_indices[k], reinterpret_cast<const double2 *>(d_data[k]),
-                          reinterpret_cast<const double2 *>(d_x[k]),
-                          reinterpret_cast<double2 *>(d_y[k]), N, G, plan.mv_items};
-            plan.mv_items += (N + kMvRows - 1) / kMvRows;
-            plan.mv.push_back(m);
-        }
+        add_item(class_of(nch),
+                 TileOp{t->gx, t->gterm, t->tz, t->tc, t->rl_ptr, t->rl, has_csr ? d_indices[k] : nullptr, dat,
+                        ip, xk, yk, 0, N, G, jw, nch, dbuf, nullptr, 0, 0, 0, G},
+                 ((N - 1) >> (10 - jw + 4)) + 1, sb);
     }
     for (int c = 0; c < kClasses; ++c) {
         plan.cls_begin[c] = uint32_t(plan.desc.size());
@@ -336,19 +365,19 @@ int make_plan(qr_batch *b, uint64_t *const *d_indptr, uint64_t *const *d_indices
     return QR_OK;
 }
 
-template <bool kNarrow, int kChunks>
+template <bool kNarrow, int kChunks, int kMode = kTilePlain>
 static void launch_class(qr_batch *b, const Plan &plan, int c, uint32_t grid_cap, uint32_t tpi,
                          hipStream_t s) {
     const uint64_t items = plan.cls_items[c];
     if (items == 0) return;
     const uint64_t grid = std::min<uint64_t>(grid_cap, items);
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, s, b->d_counter + c);
-    if (plan.split)
-        hipLaunchKernelGGL((k_batch<kNarrow, kChunks, false>), dim3(unsigned(grid)),
+    if (plan.split || kMode != kTilePlain)
+        hipLaunchKernelGGL((k_batch<kNarrow, kChunks, false, kMode>), dim3(unsigned(grid)),
                            dim3(kStageThreads), plan.cls_shmem[c], s, b->d_desc + plan.cls_begin[c],
                            plan.cls_begin[c + 1] - plan.cls_begin[c], items, tpi,
                            b->d_counter + c);
-    else
+    else if constexpr (kMode == kTilePlain)
         hipLaunchKernelGGL((k_batch<kNarrow, kChunks, true>), dim3(unsigned(grid)),
                            dim3(kStageThreads), plan.cls_shmem[c], s, b->d_desc + plan.cls_begin[c],
                            plan.cls_begin[c + 1] - plan.cls_begin[c], items, tpi,
@@ -365,9 +394,13 @@ int record(qr_batch *b, const Plan &plan, const qr_tuning &tu, uint64_t *const *
     launch_class<true, 2>(b, plan, 1, cap, tpi, s);
     launch_class<true, kStageMaxChunks>(b, plan, 2, cap, tpi, s);
     launch_class<true, kStageLeanChunks>(b, plan, 3, cap, tpi, s);
+    if (plan.split) {
+        launch_class<true, 1, kTileSub>(b, plan, kSubClass, cap, tpi, s);
+        launch_class<true, 2, kTileSub>(b, plan, kSubClass + 1, cap, tpi, s);
+    }
     QR_HIP_CHECK(hipGetLastError());
-    // operators built by the single-operator kernels (before the matvec pass
-    // that reads their CSR in the split form)
+    // fused form: operators of more than kStageMaxGLean groups, built and
+    // applied by the single-operator kernels
     for (uint32_t k : plan.big) {
         const qr_op *op = b->ops[k];
         DeviceTables *t;
@@ -376,11 +409,8 @@ int record(qr_batch *b, const Plan &plan, const qr_tuning &tu, uint64_t *const *
         const uint64_t N = uint64_t(1) << op->nq;
         rc = launch_csr_build(op, t, 0, N, d_indptr[k], d_indices[k], d_data[k], s);
         if (rc) return rc;
-        if (!plan.split) {
-            rc = launch_csr_matvec(N, N, N * op->G, d_indptr[k], d_indices[k], d_data[k], d_x[k],
-                                   d_y[k], s);
-            if (rc) return rc;
-        }
+        rc = launch_csr_matvec(N, N, N * op->G, d_indptr[k], d_indices[k], d_data[k], d_x[k], d_y[k], s);
+        if (rc) return rc;
     }
     if (plan.split && plan.mv_items) {
         // 8 resident 256-thread blocks per CU and slot
@@ -405,7 +435,9 @@ int prepare(qr_batch *b, const Plan &plan) {
         QR_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
         b->ncu = prop.multiProcessorCount;
         const int maxsh = int(stage_bytes(kStageMaxGLean, 10, nullptr));
-        const void *kerns[2 * kClasses] = {
+        // every kernel launch_class may launch (the sub-operator classes run
+        // in the split form only)
+        const void *kerns[] = {
             reinterpret_cast<const void *>(k_batch<true, 1, true>),
             reinterpret_cast<const void *>(k_batch<true, 2, true>),
             reinterpret_cast<const void *>(k_batch<true, kStageMaxChunks, true>),
@@ -413,7 +445,9 @@ int prepare(qr_batch *b, const Plan &plan) {
             reinterpret_cast<const void *>(k_batch<true, 1, false>),
             reinterpret_cast<const void *>(k_batch<true, 2, false>),
             reinterpret_cast<const void *>(k_batch<true, kStageMaxChunks, false>),
-            reinterpret_cast<const void *>(k_batch<true, kStageLeanChunks, false>)};
+            reinterpret_cast<const void *>(k_batch<true, kStageLeanChunks, false>),
+            reinterpret_cast<const void *>(k_batch<true, 1, false, kTileSub>),
+            reinterpret_cast<const void *>(k_batch<true, 2, false, kTileSub>)};
         for (const void *kf : kerns)
             QR_HIP_CHECK(hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, maxsh));
     }
