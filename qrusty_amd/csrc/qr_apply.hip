@@ -40,11 +40,10 @@
 //   FP64 : complex128 state, FP64 FMAs (tile 2^10 rows, 4 rows per thread).
 //   MIXED: complex64 state (tile 2^11 rows, 8 rows per thread).  D_g(r) is
 //          merged in FP64 -- where terms sharing an x mask cancel -- and then
-//          rounded to FP32.  Products and the sum over groups run in FP32:
-//          products are accumulated into a block partial over up to
-//          kCompBlock groups, and block partials are added to the running sum
-//          with TwoSum (s + e == a + b exactly), the error terms collected in a
-//          separate compensation.  <x|y> is reduced in FP64 from s + c.
+//          rounded to FP32.  Products run in FP32 into a block partial over up
+//          to kCompBlock groups; block partials are added into an FP64 running
+//          sum (the registers of an FP32 sum and its compensation).  y is the
+//          running sum rounded to FP32; <x|y> is reduced in FP64 from it.
 // <x|y> is reduced deterministically: per thread, per block (fixed tree),
 // then one pass over the block partials of each vector.
 #include "qr_internal.h"
@@ -65,12 +64,15 @@ namespace qr {
 // although each halving of the tile adds a far group.
 constexpr int kApplyThreads = 256;
 constexpr int kApplyThreadBits = 8;
-// Groups per FP32 block partial between TwoSum steps.  Measured on the same
-// 24-qubit Heisenberg case (8 states, psi rounded to complex64; error of
-// <psi|H|psi> relative to the FP64 value on the rounded state, max over
-// states), by block size: 1: 2.6e-7, 2.16 ms; 4: 2.8e-7, 1.69 ms;
-// 16: 5.9e-7, 1.62 ms; 32: 6.6e-7, 1.62 ms.  4 keeps the whole-call error of
-// <psi|H|psi> (rounding of psi included) near 5e-7.
+// Groups per FP32 block partial before it is added into the FP64 sum.
+// Measured on the 24-qubit Heisenberg case (8 states, psi rounded to
+// complex64; error of <psi|H|psi> relative to the FP64 value on the rounded
+// state, max over states), by block size: 1: 2.6e-7, 1.83 ms; 2: 4.2e-7,
+// 1.69 ms; 4: 2.8e-7, 1.63 ms; with an FP32 sum and TwoSum compensation
+// instead of the FP64 sum (same errors): 4: 1.69 ms, 16: 5.9e-7, 1.62 ms.
+// 4 keeps the whole-call error of <psi|H|psi> (rounding of psi included) near
+// 5e-7; the 28-qubit, 10000-term k-local apply takes 239 ms at 4 (TwoSum: 249
+// at 4, 227 at 16).
 constexpr int kCompBlock = 4;
 constexpr size_t kApplyMaxLDS = 40 * 1024;  // LDS of one block
 
@@ -96,14 +98,6 @@ template <bool kNarrow>
 __device__ __forceinline__ uint32_t parity_of(uint64_t a, uint64_t b) {
     if (kNarrow) return __popc(uint32_t(a) & uint32_t(b)) & 1u;
     return __popcll(a & b) & 1u;
-}
-
-// Knuth's TwoSum: s + e == a + b exactly (round-to-nearest, no reassociation).
-__device__ __forceinline__ void two_sum(float a, float b, float &s, float &e) {
-    s = a + b;
-    const float bp = s - a;
-    const float ap = s - bp;
-    e = (a - ap) + (b - bp);
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -468,25 +462,23 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_tiled(
 
     for (uint64_t i = tid; i < tile_rows; i += kApplyThreads) xt[i] = xv[base + i];
 
-    // accumulators: FP64 (ar, ai) or FP32 running sum s, compensation c, block partial b
+    // accumulators: the FP64 sums (ar, ai); in mixed precision the products
+    // go to the FP32 block partials (br, bi), added into ar, ai every
+    // kCompBlock groups
     double ar[R], ai[R];
-    float sr[R], si[R], cr[R], ci[R], br[R], bi[R];
+    float br[R], bi[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         ar[k] = ai[k] = 0.0;
-        sr[k] = si[k] = cr[k] = ci[k] = br[k] = bi[k] = 0.0f;
+        br[k] = bi[k] = 0.0f;
     }
 
     uint32_t since_flush = 0;  // groups in the FP32 block partials
-    // add the FP32 block partials into the running sums with TwoSum
     auto flush = [&]() {
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            float e;
-            two_sum(sr[k], br[k], sr[k], e);
-            cr[k] += e;
-            two_sum(si[k], bi[k], si[k], e);
-            ci[k] += e;
+            ar[k] += double(br[k]);
+            ai[k] += double(bi[k]);
             br[k] = bi[k] = 0.0f;
         }
     };
@@ -544,27 +536,21 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_tiled(
                                      (uint64_t(k >> 1) << (kApplyThreadBits + 1)))
                                   : tid + uint64_t(k) * kApplyThreads;
         if (!kFull && l >= tile_rows) break;
-        double yr, yi;
+        const double yr = ar[k], yi = ai[k];
         if (!kMixed) {
-            yr = ar[k];
-            yi = ai[k];
             if (yv) store_y(reinterpret_cast<double2 *>(yv) + base + l, make_double2(yr, yi));
         } else {
-            yr = double(sr[k]) + double(cr[k]);
-            yi = double(si[k]) + double(ci[k]);
             if (yv) {
                 if constexpr (kPairs) {
                     // the pair's two rows in one 16-byte store, at the odd row
                     if (k & 1) {
                         typedef float f4v __attribute__((ext_vector_type(4)));
-                        const f4v pv = {sr[k - 1] + cr[k - 1], si[k - 1] + ci[k - 1],
-                                        sr[k] + cr[k], si[k] + ci[k]};
+                        const f4v pv = {float(ar[k - 1]), float(ai[k - 1]), float(yr), float(yi)};
                         __builtin_nontemporal_store(
                             pv, reinterpret_cast<f4v *>(reinterpret_cast<float2 *>(yv) + base + l - 1));
                     }
                 } else {
-                    store_y(reinterpret_cast<float2 *>(yv) + base + l,
-                            make_float2(sr[k] + cr[k], si[k] + ci[k]));
+                    store_y(reinterpret_cast<float2 *>(yv) + base + l, make_float2(float(yr), float(yi)));
                 }
             }
         }
@@ -668,22 +654,20 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_staged(
         }
     };
 
+    // FP64 sums; mixed: FP32 block partials added in every kCompBlock groups
     double ar[R], ai[R];
-    float sr[R], si[R], cr[R], ci[R], br[R], bi[R];
+    float br[R], bi[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         ar[k] = ai[k] = 0.0;
-        sr[k] = si[k] = cr[k] = ci[k] = br[k] = bi[k] = 0.0f;
+        br[k] = bi[k] = 0.0f;
     }
     uint32_t since_flush = 0;
     auto flush = [&]() {
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            float e;
-            two_sum(sr[k], br[k], sr[k], e);
-            cr[k] += e;
-            two_sum(si[k], bi[k], si[k], e);
-            ci[k] += e;
+            ar[k] += double(br[k]);
+            ai[k] += double(bi[k]);
             br[k] = bi[k] = 0.0f;
         }
     };
@@ -746,15 +730,11 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_staged(
     for (int k = 0; k < R; ++k) {
         const uint64_t l = tid + uint64_t(k) * kApplyThreads;
         if (!kFull && l >= tile_rows) break;
-        double yr, yi;
+        const double yr = ar[k], yi = ai[k];
         if (!kMixed) {
-            yr = ar[k];
-            yi = ai[k];
             if (yv) store_y(reinterpret_cast<double2 *>(yv) + base + l, make_double2(yr, yi));
         } else {
-            yr = double(sr[k]) + double(cr[k]);
-            yi = double(si[k]) + double(ci[k]);
-            if (yv) store_y(reinterpret_cast<float2 *>(yv) + base + l, make_float2(sr[k] + cr[k], si[k] + ci[k]));
+            if (yv) store_y(reinterpret_cast<float2 *>(yv) + base + l, make_float2(float(yr), float(yi)));
         }
         if (partials) {
             const cT xr = xv[base + l];
