@@ -122,6 +122,33 @@ def test_batch_vs_oracle_and_replays(batch_case):
         assert np.array_equal(b.y[k].cpu().numpy(), ys[0][k])
 
 
+@pytest.mark.parametrize("split", [1, 0])
+def test_batch_heavy_groups(split):
+    """Operators with a heavy diagonal group (60-150 Z strings) beside random
+    terms: one of more than 2048 groups (sub-operators in the split form), one
+    of 1025-2048 and one of at most 1024 groups.  The split form folds heavy
+    groups inline in its sub-operator kernels; CSR bit-exact, y at 1e-12."""
+    import qrusty_amd as qa
+    rng = np.random.default_rng(4242)
+    ops, params = [], []
+    for n, T, nd in [(13, 3200, 150), (11, 1500, 60), (12, 500, 100)]:
+        labels = ["".join(rng.choice(list("IXYZ"), size=n)) for _ in range(T)]
+        labels += ["".join(rng.choice(["I", "Z"], size=n)) for _ in range(nd)]
+        coeffs = rng.standard_normal(len(labels)) + 1j * rng.standard_normal(len(labels))
+        ops.append(qa.SparsePauliOp([qa.Pauli(l) for l in labels], coeffs))
+        params.append(orc.Params(labels, coeffs))
+    Gs = [len(np.unique(p.x)) for p in params]
+    print(f"[gpu] batch heavy groups split={split} G={Gs}", flush=True)
+    assert Gs[0] > 2048 and 1024 < Gs[1] <= 2048 and Gs[2] <= 1024
+    b = qa.Batch(ops, keep_csr=True)
+    xs = _states(params, seed=31)
+    b.set_states(xs)
+    b.run({"tiles_per_item": 2, "blocks_per_cu": 2, "order": 0, "split": split})
+    torch.cuda.synchronize()
+    for k, p in enumerate(params):
+        _check_op(b, k, p, xs[k])
+
+
 def test_batch_tunings_bit_identical(batch_case):
     import qrusty_amd as qa
     ops, params = batch_case
