@@ -314,6 +314,40 @@ def test_matvec_long_rows_ragged(rows, cols, unsorted):
     _check_matvec(y, ref, A.dot(np.abs(x)))
 
 
+@pytest.mark.parametrize("avg", [20, 100, 400])
+def test_matvec_malformed_csr_skips_out_of_range(avg):
+    """A CSR handed in through new_unchecked with columns >= ncols (some near
+    2^64) and indptr entries beyond nnz: every matvec kernel (16-32, fewer
+    than 256 and 256+ entries per row on average) skips those columns and
+    clamps the rows to [0, nnz), so y is the sum over the remaining entries."""
+    import qrusty_amd as qa
+    rng = np.random.default_rng(900 + avg)
+    rows, cols = 333, 5000
+    lens = rng.integers(avg // 2, avg + avg // 2 + 1, size=rows)
+    indptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint64)
+    nnz = int(indptr[-1])
+    indices = rng.integers(0, cols, size=nnz).astype(np.uint64)
+    bad = rng.random(nnz) < 0.05
+    indices[bad] = cols + rng.integers(0, 1000, size=int(bad.sum())).astype(np.uint64)
+    indices[rng.random(nnz) < 0.01] = np.uint64(2**64 - 7)
+    indptr[rows - 20:] += np.uint64(10**6)  # the last rows end (and start) beyond nnz
+    data = rng.standard_normal(nnz) + 1j * rng.standard_normal(nnz)
+    x = rng.standard_normal(cols) + 1j * rng.standard_normal(cols)
+    print(f"[gpu] malformed CSR avg={avg} nnz={nnz}", flush=True)
+    m = qa.SpMat.new_unchecked((rows, cols), data, indices, indptr)
+    y = qa.spmat_dot_densevec(m, x)
+    ref = np.zeros(rows, dtype=np.complex128)
+    absb = np.zeros(rows)
+    for r in range(rows):
+        k0, k1 = min(int(indptr[r]), nnz), min(int(indptr[r + 1]), nnz)
+        sl = slice(k0, max(k0, k1))
+        ok = indices[sl] < cols
+        c = indices[sl][ok].astype(np.int64)
+        ref[r] = np.sum(data[sl][ok] * x[c])
+        absb[r] = np.sum(np.abs(data[sl][ok]) * np.abs(x[c]))
+    assert np.all(np.abs(y - ref) <= 1e-12 * absb + 1e-300)
+
+
 def test_device_tensor_matvec_stays_on_device(hamiltonians):
     import qrusty_amd as qa
     m = _op(*hamiltonians["py_H4"]).to_matrix_mode("Hip")
