@@ -61,7 +61,12 @@ namespace qr {
 //   1024, 2^13 / 2^12 rows (mixed / FP64): 1.80 / 3.75 ms; 252 / 422 ms
 //    512, 2^12 / 2^11 rows:                1.75 / 3.66 ms; 237 / 398 ms
 //    256, 2^11 / 2^10 rows:                1.64 / 3.47 ms; 228 / 400 ms
-// although each halving of the tile adds a far group.
+// although each halving of the tile adds a far group.  Fewer rows per thread
+// on the same tiles (512 threads, 2^11 / 2^10 rows: 4 / 2 rows per thread,
+// 72-88 VGPRs, 5-7 waves per SIMD) measured 1.84 / 3.59 ms and 278 / 436 ms
+// against 1.63 / 3.50 and 238 / 401 (FP64 sums of FP32 blocks): the per-group
+// work of a thread is shared by fewer rows.  Forcing 5 or 6 waves per SIMD
+// on 256 threads spills 120-330 registers.
 constexpr int kApplyThreads = 256;
 constexpr int kApplyThreadBits = 8;
 // Groups per FP32 block partial before it is added into the FP64 sum.
