@@ -292,8 +292,8 @@ int qr_mm_read_close(qr_mm_reader *reader);
  * for nvec state vectors stored one after another (x_v at offset v*2^n).
  *   QR_PREC_FP64  : x, y complex128; FP64 throughout.
  *   QR_PREC_MIXED : x, y complex64; D_g(r) merged in FP64 and rounded to FP32,
- *                   products summed in FP32 over blocks of 4 groups, the
- *                   blocks in FP64; <x|y> reduced in FP64.
+ *                   products and the sum over groups in FP32 with TwoSum
+ *                   compensation; <x|y> reduced in FP64.
  * d_expect (optional, may be NULL): 2*nvec doubles (device) receiving
  * <x_v | y_v> = sum_r conj(x_v[r]) y_v[r] in FP64.
  * d_y may be NULL when d_expect is given: then only <x|H|x> is produced and

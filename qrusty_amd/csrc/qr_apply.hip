@@ -40,10 +40,11 @@
 //   FP64 : complex128 state, FP64 FMAs (tile 2^10 rows, 4 rows per thread).
 //   MIXED: complex64 state (tile 2^11 rows, 8 rows per thread).  D_g(r) is
 //          merged in FP64 -- where terms sharing an x mask cancel -- and then
-//          rounded to FP32.  Products run in FP32 into a block partial over up
-//          to kCompBlock groups; block partials are added into an FP64 running
-//          sum (the registers of an FP32 sum and its compensation).  y is the
-//          running sum rounded to FP32; <x|y> is reduced in FP64 from it.
+//          rounded to FP32.  Products and the sum over groups run in FP32:
+//          products are accumulated into a block partial over up to
+//          kCompBlock groups, and block partials are added to the running sum
+//          with TwoSum (s + e == a + b exactly), the error terms collected in a
+//          separate compensation.  <x|y> is reduced in FP64 from s + c.
 // <x|y> is reduced deterministically: per thread, per block (fixed tree),
 // then one pass over the block partials of each vector.
 #include "qr_internal.h"
@@ -64,20 +65,21 @@ namespace qr {
 // although each halving of the tile adds a far group.  Fewer rows per thread
 // on the same tiles (512 threads, 2^11 / 2^10 rows: 4 / 2 rows per thread,
 // 72-88 VGPRs, 5-7 waves per SIMD) measured 1.84 / 3.59 ms and 278 / 436 ms
-// against 1.63 / 3.50 and 238 / 401 (FP64 sums of FP32 blocks): the per-group
-// work of a thread is shared by fewer rows.  Forcing 5 or 6 waves per SIMD
-// on 256 threads spills 120-330 registers.
+// against 1.63 / 3.50 and 238 / 401 with 256 threads (block partials summed
+// in FP64 in both): the per-group work of a thread is shared by fewer rows.
+// Forcing 5 or 6 waves per SIMD on 256 threads spills 120-330 registers.
 constexpr int kApplyThreads = 256;
 constexpr int kApplyThreadBits = 8;
-// Groups per FP32 block partial before it is added into the FP64 sum.
-// Measured on the 24-qubit Heisenberg case (8 states, psi rounded to
-// complex64; error of <psi|H|psi> relative to the FP64 value on the rounded
-// state, max over states), by block size: 1: 2.6e-7, 1.83 ms; 2: 4.2e-7,
-// 1.69 ms; 4: 2.8e-7, 1.63 ms; with an FP32 sum and TwoSum compensation
-// instead of the FP64 sum (same errors): 4: 1.69 ms, 16: 5.9e-7, 1.62 ms.
-// 4 keeps the whole-call error of <psi|H|psi> (rounding of psi included) near
-// 5e-7; the 28-qubit, 10000-term k-local apply takes 239 ms at 4 (TwoSum: 249
-// at 4, 227 at 16).
+// Groups per FP32 block partial between TwoSum steps.  Measured on the same
+// 24-qubit Heisenberg case (8 states, psi rounded to complex64; error of
+// <psi|H|psi> relative to the FP64 value on the rounded state, max over
+// states), by block size: 1: 2.6e-7, 2.16 ms; 4: 2.8e-7, 1.69 ms;
+// 16: 5.9e-7, 1.62 ms; 32: 6.6e-7, 1.62 ms.  4 keeps the whole-call error of
+// <psi|H|psi> (rounding of psi included) near 5e-7.  Adding the block
+// partials into FP64 running sums instead (the registers of s and c) gave
+// the same errors at 1.63 ms, and 239 ms instead of 249 on the 28-qubit
+// k-local apply; the accumulation stays FP32 with TwoSum compensation, FP64
+// being kept to the merging of coefficients and the <x|y> reduction.
 constexpr int kCompBlock = 4;
 constexpr size_t kApplyMaxLDS = 40 * 1024;  // LDS of one block
 
@@ -103,6 +105,14 @@ template <bool kNarrow>
 __device__ __forceinline__ uint32_t parity_of(uint64_t a, uint64_t b) {
     if (kNarrow) return __popc(uint32_t(a) & uint32_t(b)) & 1u;
     return __popcll(a & b) & 1u;
+}
+
+// Knuth's TwoSum: s + e == a + b exactly (round-to-nearest, no reassociation).
+__device__ __forceinline__ void two_sum(float a, float b, float &s, float &e) {
+    s = a + b;
+    const float bp = s - a;
+    const float ap = s - bp;
+    e = (a - ap) + (b - bp);
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -467,23 +477,25 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_tiled(
 
     for (uint64_t i = tid; i < tile_rows; i += kApplyThreads) xt[i] = xv[base + i];
 
-    // accumulators: the FP64 sums (ar, ai); in mixed precision the products
-    // go to the FP32 block partials (br, bi), added into ar, ai every
-    // kCompBlock groups
+    // accumulators: FP64 (ar, ai) or FP32 running sum s, compensation c, block partial b
     double ar[R], ai[R];
-    float br[R], bi[R];
+    float sr[R], si[R], cr[R], ci[R], br[R], bi[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         ar[k] = ai[k] = 0.0;
-        br[k] = bi[k] = 0.0f;
+        sr[k] = si[k] = cr[k] = ci[k] = br[k] = bi[k] = 0.0f;
     }
 
     uint32_t since_flush = 0;  // groups in the FP32 block partials
+    // add the FP32 block partials into the running sums with TwoSum
     auto flush = [&]() {
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            ar[k] += double(br[k]);
-            ai[k] += double(bi[k]);
+            float e;
+            two_sum(sr[k], br[k], sr[k], e);
+            cr[k] += e;
+            two_sum(si[k], bi[k], si[k], e);
+            ci[k] += e;
             br[k] = bi[k] = 0.0f;
         }
     };
@@ -541,21 +553,27 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_tiled(
                                      (uint64_t(k >> 1) << (kApplyThreadBits + 1)))
                                   : tid + uint64_t(k) * kApplyThreads;
         if (!kFull && l >= tile_rows) break;
-        const double yr = ar[k], yi = ai[k];
+        double yr, yi;
         if (!kMixed) {
+            yr = ar[k];
+            yi = ai[k];
             if (yv) store_y(reinterpret_cast<double2 *>(yv) + base + l, make_double2(yr, yi));
         } else {
+            yr = double(sr[k]) + double(cr[k]);
+            yi = double(si[k]) + double(ci[k]);
             if (yv) {
                 if constexpr (kPairs) {
                     // the pair's two rows in one 16-byte store, at the odd row
                     if (k & 1) {
                         typedef float f4v __attribute__((ext_vector_type(4)));
-                        const f4v pv = {float(ar[k - 1]), float(ai[k - 1]), float(yr), float(yi)};
+                        const f4v pv = {sr[k - 1] + cr[k - 1], si[k - 1] + ci[k - 1],
+                                        sr[k] + cr[k], si[k] + ci[k]};
                         __builtin_nontemporal_store(
                             pv, reinterpret_cast<f4v *>(reinterpret_cast<float2 *>(yv) + base + l - 1));
                     }
                 } else {
-                    store_y(reinterpret_cast<float2 *>(yv) + base + l, make_float2(float(yr), float(yi)));
+                    store_y(reinterpret_cast<float2 *>(yv) + base + l,
+                            make_float2(sr[k] + cr[k], si[k] + ci[k]));
                 }
             }
         }
@@ -659,20 +677,22 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_staged(
         }
     };
 
-    // FP64 sums; mixed: FP32 block partials added in every kCompBlock groups
     double ar[R], ai[R];
-    float br[R], bi[R];
+    float sr[R], si[R], cr[R], ci[R], br[R], bi[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         ar[k] = ai[k] = 0.0;
-        br[k] = bi[k] = 0.0f;
+        sr[k] = si[k] = cr[k] = ci[k] = br[k] = bi[k] = 0.0f;
     }
     uint32_t since_flush = 0;
     auto flush = [&]() {
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            ar[k] += double(br[k]);
-            ai[k] += double(bi[k]);
+            float e;
+            two_sum(sr[k], br[k], sr[k], e);
+            cr[k] += e;
+            two_sum(si[k], bi[k], si[k], e);
+            ci[k] += e;
             br[k] = bi[k] = 0.0f;
         }
     };
@@ -735,11 +755,15 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_staged(
     for (int k = 0; k < R; ++k) {
         const uint64_t l = tid + uint64_t(k) * kApplyThreads;
         if (!kFull && l >= tile_rows) break;
-        const double yr = ar[k], yi = ai[k];
+        double yr, yi;
         if (!kMixed) {
+            yr = ar[k];
+            yi = ai[k];
             if (yv) store_y(reinterpret_cast<double2 *>(yv) + base + l, make_double2(yr, yi));
         } else {
-            if (yv) store_y(reinterpret_cast<float2 *>(yv) + base + l, make_float2(float(yr), float(yi)));
+            yr = double(sr[k]) + double(cr[k]);
+            yi = double(si[k]) + double(ci[k]);
+            if (yv) store_y(reinterpret_cast<float2 *>(yv) + base + l, make_float2(sr[k] + cr[k], si[k] + ci[k]));
         }
         if (partials) {
             const cT xr = xv[base + l];
