@@ -163,11 +163,16 @@ __device__ __forceinline__ void gather(uint64_t m, uint32_t tid, uint64_t tile_r
 #pragma unroll
             for (int j = 0; j < R / 2; ++j) pr[j] = xrow[(uint32_t(j) ^ jm) << kApplyThreadBits];
         }
+        // the swap is wave-uniform and rare (masks with row bit 0): a scalar
+        // branch, so that the other groups spend no VALU selects on it
+        if (__builtin_expect(swap, 0)) {
+#pragma unroll
+            for (int j = 0; j < R / 2; ++j) pr[j] = make_float4(pr[j].z, pr[j].w, pr[j].x, pr[j].y);
+        }
 #pragma unroll
         for (int j = 0; j < R / 2; ++j) {
-            const float2 lo = make_float2(pr[j].x, pr[j].y), hi = make_float2(pr[j].z, pr[j].w);
-            xg[2 * j] = swap ? hi : lo;
-            xg[2 * j + 1] = swap ? lo : hi;
+            xg[2 * j] = make_float2(pr[j].x, pr[j].y);
+            xg[2 * j + 1] = make_float2(pr[j].z, pr[j].w);
         }
         return;
     }
