@@ -43,8 +43,9 @@
 //          rounded to FP32.  Products and the sum over groups run in FP32:
 //          products are accumulated into a block partial over up to
 //          kCompBlock groups, and block partials are added to the running sum
-//          with TwoSum (s + e == a + b exactly), the error terms collected in a
-//          separate compensation.  <x|y> is reduced in FP64 from s + c.
+//          with TwoSum (s + e == a + b exactly); the error e starts the next
+//          block partial, so y = s + b at the end.  <x|y> is reduced in FP64
+//          from s + b.
 // <x|y> is reduced deterministically: per thread, per block (fixed tree),
 // then one pass over the block partials of each vector.
 #include "qr_internal.h"
@@ -75,11 +76,14 @@ constexpr int kApplyThreadBits = 8;
 // <psi|H|psi> relative to the FP64 value on the rounded state, max over
 // states), by block size: 1: 2.6e-7, 2.16 ms; 4: 2.8e-7, 1.69 ms;
 // 16: 5.9e-7, 1.62 ms; 32: 6.6e-7, 1.62 ms.  4 keeps the whole-call error of
-// <psi|H|psi> (rounding of psi included) near 5e-7.  Adding the block
-// partials into FP64 running sums instead (the registers of s and c) gave
-// the same errors at 1.63 ms, and 239 ms instead of 249 on the 28-qubit
-// k-local apply; the accumulation stays FP32 with TwoSum compensation, FP64
-// being kept to the merging of coefficients and the <x|y> reduction.
+// <psi|H|psi> (rounding of psi included) near 5e-7.  Those figures are
+// with the TwoSum errors collected in a separate compensation c; feeding
+// each error into the next block partial instead saves the compensation's
+// adds, zeroing and 16 registers (113 -> 95 VGPRs, 5 waves per SIMD): 1.651
+// -> 1.583 ms, E 5.05e-7 -> 5.22e-7 end to end, y unchanged.  Adding the
+// block partials into FP64 running sums gave 1.63 ms; the accumulation stays
+// FP32 with TwoSum compensation, FP64 being kept to the merging of
+// coefficients and the <x|y> reduction.
 constexpr int kCompBlock = 4;
 constexpr size_t kApplyMaxLDS = 40 * 1024;  // LDS of one block
 
@@ -484,11 +488,11 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_tiled(
 
     // accumulators: FP64 (ar, ai) or FP32 running sum s, compensation c, block partial b
     double ar[R], ai[R];
-    float sr[R], si[R], cr[R], ci[R], br[R], bi[R];
+    float sr[R], si[R], br[R], bi[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         ar[k] = ai[k] = 0.0;
-        sr[k] = si[k] = cr[k] = ci[k] = br[k] = bi[k] = 0.0f;
+        sr[k] = si[k] = br[k] = bi[k] = 0.0f;
     }
 
     uint32_t since_flush = 0;  // groups in the FP32 block partials
@@ -496,12 +500,11 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_tiled(
     auto flush = [&]() {
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            float e;
+            float e, f;
             two_sum(sr[k], br[k], sr[k], e);
-            cr[k] += e;
-            two_sum(si[k], bi[k], si[k], e);
-            ci[k] += e;
-            br[k] = bi[k] = 0.0f;
+            two_sum(si[k], bi[k], si[k], f);
+            br[k] = e;  // the rounding error starts the next block partial
+            bi[k] = f;
         }
     };
     cT xg[R];
@@ -564,21 +567,21 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_tiled(
             yi = ai[k];
             if (yv) store_y(reinterpret_cast<double2 *>(yv) + base + l, make_double2(yr, yi));
         } else {
-            yr = double(sr[k]) + double(cr[k]);
-            yi = double(si[k]) + double(ci[k]);
+            yr = double(sr[k]) + double(br[k]);
+            yi = double(si[k]) + double(bi[k]);
             if (yv) {
                 if constexpr (kPairs) {
                     // the pair's two rows in one 16-byte store, at the odd row
                     if (k & 1) {
                         typedef float f4v __attribute__((ext_vector_type(4)));
-                        const f4v pv = {sr[k - 1] + cr[k - 1], si[k - 1] + ci[k - 1],
-                                        sr[k] + cr[k], si[k] + ci[k]};
+                        const f4v pv = {sr[k - 1] + br[k - 1], si[k - 1] + bi[k - 1],
+                                        sr[k] + br[k], si[k] + bi[k]};
                         __builtin_nontemporal_store(
                             pv, reinterpret_cast<f4v *>(reinterpret_cast<float2 *>(yv) + base + l - 1));
                     }
                 } else {
                     store_y(reinterpret_cast<float2 *>(yv) + base + l,
-                            make_float2(sr[k] + cr[k], si[k] + ci[k]));
+                            make_float2(sr[k] + br[k], si[k] + bi[k]));
                 }
             }
         }
@@ -683,22 +686,21 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_staged(
     };
 
     double ar[R], ai[R];
-    float sr[R], si[R], cr[R], ci[R], br[R], bi[R];
+    float sr[R], si[R], br[R], bi[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         ar[k] = ai[k] = 0.0;
-        sr[k] = si[k] = cr[k] = ci[k] = br[k] = bi[k] = 0.0f;
+        sr[k] = si[k] = br[k] = bi[k] = 0.0f;
     }
     uint32_t since_flush = 0;
     auto flush = [&]() {
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            float e;
+            float e, f;
             two_sum(sr[k], br[k], sr[k], e);
-            cr[k] += e;
-            two_sum(si[k], bi[k], si[k], e);
-            ci[k] += e;
-            br[k] = bi[k] = 0.0f;
+            two_sum(si[k], bi[k], si[k], f);
+            br[k] = e;  // the rounding error starts the next block partial
+            bi[k] = f;
         }
     };
     cT xg[R];
@@ -766,9 +768,9 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_staged(
             yi = ai[k];
             if (yv) store_y(reinterpret_cast<double2 *>(yv) + base + l, make_double2(yr, yi));
         } else {
-            yr = double(sr[k]) + double(cr[k]);
-            yi = double(si[k]) + double(ci[k]);
-            if (yv) store_y(reinterpret_cast<float2 *>(yv) + base + l, make_float2(sr[k] + cr[k], si[k] + ci[k]));
+            yr = double(sr[k]) + double(br[k]);
+            yi = double(si[k]) + double(bi[k]);
+            if (yv) store_y(reinterpret_cast<float2 *>(yv) + base + l, make_float2(sr[k] + br[k], si[k] + bi[k]));
         }
         if (partials) {
             const cT xr = xv[base + l];
