@@ -711,7 +711,7 @@ def csr_matrix(m: SpMat):
 # ---------------------------------------------------------------------------
 class Batch:
     """Several operators built into CSR and applied to one state each, in one
-    graph launch of the persistent batch kernel (qr_batch_run).
+    graph launch of the persistent batch kernels (qr_batch_run).
 
     Output buffers are allocated once and reused by every run(), so the
     recorded graph is replayed; CSR outputs are kept only when keep_csr

@@ -1,10 +1,12 @@
 // qr_batch.hip -- many operators of uneven size: CSR builds plus one matvec
-// each, in one persistent launch captured in a hipGraph, with an autotuner.
+// each, as a few persistent kernels captured in one hipGraph, with an
+// autotuner.
 //
 // The reference builds and applies operators one at a time
 // (qrusty/src/lib.rs:573-585 then accel.rs:338-370).  Here the work of a whole
-// batch is a single list of (operator, row tile) items.  A persistent grid of
-// 1024-thread blocks takes items from one atomic counter; an item builds its
+// batch is lists of (operator, row tile) items, one per register class.  A
+// persistent grid of 1024-thread blocks takes items from its class's atomic
+// counter; an item builds its
 // rows with the shared LDS-staged tile builder (qr_tile.h) -- CSR entries
 // bit-identical to the single-operator build -- and, fused, the same rows of
 // y = A x from the entries still in registers.  No item depends on another,
