@@ -131,15 +131,15 @@ void build_patterns(const qr_op *op, int L, int tb, uint32_t pcap, uint32_t ucap
     // closes the chunk: its groups are reordered by class, or (staged) by
     // partner tile and then class (patterns stay where they are; the table
     // rows follow their groups)
-    auto close_chunk = [&out, &cur, far_mask, staged]() {
+    auto close_chunk = [&out, &cur, far_mask, staged, pairs]() {
         const uint32_t g0 = cur.cg[0], g1 = uint32_t(out.grp.size());
         cur.p1 = uint32_t(out.pzl.size());
         cur.u1 = uint32_t(out.ug.size());
         cur.r0 = cur.r1 = uint32_t(out.run.size());
         if (g1 > g0) {
-            auto cls = [far_mask](const ApplyGroup &a) {
+            auto cls = [far_mask, pairs](const ApplyGroup &a) {
                 return ((a.m & far_mask) ? 1u : 0u) | ((a.u >= 0 || a.ps != a.pk) ? 2u : 0u) |
-                       ((a.flags & kGroupReal) ? 0u : 4u);
+                       ((a.flags & kGroupReal) ? 0u : 4u) | ((pairs && (a.m & 1)) ? 8u : 0u);
             };
             if (!staged) {
                 std::stable_sort(out.grp.begin() + g0, out.grp.end(),
@@ -150,9 +150,10 @@ void build_patterns(const qr_op *op, int L, int tb, uint32_t pcap, uint32_t ucap
                     while (c <= cg && c <= uint32_t(kApplyClasses)) cur.cg[c++] = g;
                 }
             } else {
-                // run class: bit 0 row-varying, bit 1 complex (cls without the far bit)
+                // run class: bit 0 row-varying, bit 1 complex (cls without the
+                // far bit; the staged form has no paired rows)
                 auto key = [&cls, far_mask](const ApplyGroup &a) {
-                    return std::make_pair(a.m & far_mask, cls(a) >> 1);
+                    return std::make_pair(a.m & far_mask, (cls(a) >> 1) & 3u);
                 };
                 std::stable_sort(out.grp.begin() + g0, out.grp.end(),
                                  [&key](const ApplyGroup &a, const ApplyGroup &b) { return key(a) < key(b); });

@@ -140,7 +140,9 @@ constexpr int kBufferWord3 = 0x00020000;
 // one uniform base pointer per row block and the thread's offset shared.
 // With kPairs (complex64, full tiles) the thread's rows k = 2j, 2j+1 are the
 // adjacent rows (2 tid, 2 tid + 1) + 2^9 j: their partners r ^ m form one
-// aligned pair, read with one 16-byte load and swapped when bit 0 of m is set.
+// aligned pair, read with one 16-byte load.  When bit 0 of m is set, row k's
+// partner is xg[k ^ 1]: such groups form classes of their own, whose
+// accumulation reads the pair crosswise (no data is moved).
 template <bool kFar, bool kFull, bool kPairs, int R, typename cT>
 __device__ __forceinline__ void gather(uint64_t m, uint32_t tid, uint64_t tile_rows, uint64_t base,
                                        const cT *xt, const cT *__restrict__ xv, cT (&xg)[R]) {
@@ -150,7 +152,6 @@ __device__ __forceinline__ void gather(uint64_t m, uint32_t tid, uint64_t tile_r
         constexpr int kJ = kApplyThreadBits + 1;  // row bit of the pair index j
         const uint32_t tq = tid ^ uint32_t((mlo >> 1) & (kApplyThreads - 1));
         const uint32_t jm = uint32_t(mlo >> kJ);
-        const bool swap = (mlo & 1) != 0;
         float4 pr[R / 2];
         if (kFar) {
             const cT *far = xv + (base ^ mhi);
@@ -166,12 +167,6 @@ __device__ __forceinline__ void gather(uint64_t m, uint32_t tid, uint64_t tile_r
             const float4 *xrow = reinterpret_cast<const float4 *>(xt) + tq;
 #pragma unroll
             for (int j = 0; j < R / 2; ++j) pr[j] = xrow[(uint32_t(j) ^ jm) << kApplyThreadBits];
-        }
-        // the swap is wave-uniform and rare (masks with row bit 0): a scalar
-        // branch, so that the other groups spend no VALU selects on it
-        if (__builtin_expect(swap, 0)) {
-#pragma unroll
-            for (int j = 0; j < R / 2; ++j) pr[j] = make_float4(pr[j].z, pr[j].w, pr[j].x, pr[j].y);
         }
 #pragma unroll
         for (int j = 0; j < R / 2; ++j) {
@@ -209,18 +204,23 @@ __device__ __forceinline__ void gather(uint64_t m, uint32_t tid, uint64_t tile_r
     }
 }
 
-// y += D_g(r) x[r ^ m] for the R rows of a thread, x already gathered.  D_g is
+// y += D_g(r) x[r ^ m] for the R rows of a thread.  fetch() gathers x into
+// xg (row k's partner in xg[k], or in xg[k ^ 1] with kSwap): first thing for
+// groups with one D per thread, so that the loads fly during the pattern
+// loop; after the per-row D for row-varying groups, so that the gathered
+// registers and the coefficient temporaries are not live together.  D_g is
 // formed from the chunk's pattern coefficients: thread-bit patterns once per
 // thread, then -- for row-varying groups (kRowvar) -- per row the row-bit
 // table entry sU[urow*R + k] and the straddling patterns.  Products go to the
 // FP64 accumulators ar/ai, or (mixed) to the FP32 block partials br/bi; those
 // with Im D only for complex groups (kCplx).
-template <bool kMixed, bool kFull, bool kRowvar, bool kCplx, int R, typename cT>
+template <bool kMixed, bool kFull, bool kRowvar, bool kCplx, bool kSwap, int R, typename cT, typename Fetch>
 __device__ __forceinline__ void accumulate(const ApplyGroup &gd, uint32_t tid, uint64_t tile_rows,
                                            uint32_t zero_row, const double2 *sA, const uint32_t *sZ,
-                                           const double2 *sU, const cT (&xg)[R], double (&ar)[R],
+                                           const double2 *sU, cT (&xg)[R], Fetch fetch, double (&ar)[R],
                                            double (&ai)[R], float (&br)[R], float (&bi)[R]) {
     constexpr bool cplx = kCplx;
+    if (!kRowvar) fetch();
     double cr = 0.0, ci = 0.0;
     uint32_t p = gd.p0;
     for (; p + 1 < gd.ps; p += 2) {  // two patterns per step: loads in flight together
@@ -247,34 +247,43 @@ __device__ __forceinline__ void accumulate(const ApplyGroup &gd, uint32_t tid, u
 #pragma unroll
             for (int k = 0; k < R; ++k) {
                 if (!kFull && tid + uint64_t(k) * kApplyThreads >= tile_rows) break;
-                br[k] = fmaf(fr, xg[k].x, br[k]);
-                bi[k] = fmaf(fr, xg[k].y, bi[k]);
+                br[k] = fmaf(fr, xg[k ^ int(kSwap)].x, br[k]);
+                bi[k] = fmaf(fr, xg[k ^ int(kSwap)].y, bi[k]);
             }
             if (cplx) {
 #pragma unroll
                 for (int k = 0; k < R; ++k) {
                     if (!kFull && tid + uint64_t(k) * kApplyThreads >= tile_rows) break;
-                    br[k] = fmaf(-fi, xg[k].y, br[k]);
-                    bi[k] = fmaf(fi, xg[k].x, bi[k]);
+                    br[k] = fmaf(-fi, xg[k ^ int(kSwap)].y, br[k]);
+                    bi[k] = fmaf(fi, xg[k ^ int(kSwap)].x, bi[k]);
                 }
             }
         } else {
 #pragma unroll
             for (int k = 0; k < R; ++k) {
                 if (!kFull && tid + uint64_t(k) * kApplyThreads >= tile_rows) break;
-                ar[k] = fma(cr, xg[k].x, ar[k]);
-                ai[k] = fma(cr, xg[k].y, ai[k]);
+                ar[k] = fma(cr, xg[k ^ int(kSwap)].x, ar[k]);
+                ai[k] = fma(cr, xg[k ^ int(kSwap)].y, ai[k]);
             }
             if (cplx) {
 #pragma unroll
                 for (int k = 0; k < R; ++k) {
                     if (!kFull && tid + uint64_t(k) * kApplyThreads >= tile_rows) break;
-                    ar[k] = fma(-ci, xg[k].y, ar[k]);
-                    ai[k] = fma(ci, xg[k].x, ai[k]);
+                    ar[k] = fma(-ci, xg[k ^ int(kSwap)].y, ar[k]);
+                    ai[k] = fma(ci, xg[k ^ int(kSwap)].x, ai[k]);
                 }
             }
         }
         return;
+    }
+    // D per row for the row-varying groups (rounded to FP32 in mixed mode),
+    // formed by one of the paths below; a single block of products follows,
+    // so that every path leaves the accumulators in the same registers
+    using dT = typename std::conditional<kMixed, float, double>::type;
+    dT dre[R], dim[R];
+    if (!kFull) {
+#pragma unroll
+        for (int k = 0; k < R; ++k) dre[k] = dim[k] = dT(0);
     }
     const double2 *su = sU + (gd.u >= 0 ? uint32_t(gd.u) : zero_row) * R;
     const uint32_t nst = gd.pk - gd.ps;
@@ -301,64 +310,55 @@ __device__ __forceinline__ void accumulate(const ApplyGroup &gd, uint32_t tid, u
                 double dr = cr + uk.x, di = cplx ? ci + uk.y : 0.0;
 #pragma unroll
                 for (uint32_t j = 0; j < kN; ++j) {
-                    const uint32_t s = uint32_t(__popc(uint32_t(k) & zh[j])) << 31;
-                    dr += flip_sign(aq[j].x, s);
-                    if (cplx) di += flip_sign(aq[j].y, s);
+                    // the row sign is wave-uniform: fma(+-1, a, d) rounds d +- a
+                    // exactly as the add does, with the sign in scalar registers
+                    const double sg = (__popc(uint32_t(k) & zh[j]) & 1u) ? -1.0 : 1.0;
+                    dr = fma(sg, aq[j].x, dr);
+                    if (cplx) di = fma(sg, aq[j].y, di);
                 }
-                const cT xb = xg[k];
-                if (!kMixed) {
-                    ar[k] = fma(dr, xb.x, ar[k]);
-                    ai[k] = fma(dr, xb.y, ai[k]);
-                    if (cplx) {
-                        ar[k] = fma(-di, xb.y, ar[k]);
-                        ai[k] = fma(di, xb.x, ai[k]);
-                    }
-                } else {
-                    const float fr = float(dr);
-                    br[k] = fmaf(fr, xb.x, br[k]);
-                    bi[k] = fmaf(fr, xb.y, bi[k]);
-                    if (cplx) {
-                        const float fi = float(di);
-                        br[k] = fmaf(-fi, xb.y, br[k]);
-                        bi[k] = fmaf(fi, xb.x, bi[k]);
-                    }
-                }
+                dre[k] = dT(dr);
+                if (cplx) dim[k] = dT(di);
             }
         };
         if (nst == 0) rows(std::integral_constant<uint32_t, 0>{});
         else if (nst == 1) rows(std::integral_constant<uint32_t, 1>{});
         else rows(std::integral_constant<uint32_t, 2>{});
-        return;
+    } else {
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            if (!kFull && tid + uint64_t(k) * kApplyThreads >= tile_rows) break;
+            const double2 uk = su[k];
+            double dr = cr + uk.x, di = cplx ? ci + uk.y : 0.0;
+            for (uint32_t q = gd.ps; q < gd.pk; ++q) {
+                const double2 a = sA[q];
+                const uint32_t zl = uint32_t(__builtin_amdgcn_readfirstlane(int(sZ[q])));
+                const uint32_t flip =
+                    uint32_t(__popc(tid & zl) + __popc(uint32_t(k) & (zl >> kApplyThreadBits))) << 31;
+                dr += flip_sign(a.x, flip);
+                if (cplx) di += flip_sign(a.y, flip);
+            }
+            dre[k] = dT(dr);
+            if (cplx) dim[k] = dT(di);
+        }
     }
+    fetch();
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         if (!kFull && tid + uint64_t(k) * kApplyThreads >= tile_rows) break;
-        const double2 uk = su[k];
-        double dr = cr + uk.x, di = cplx ? ci + uk.y : 0.0;
-        for (uint32_t q = gd.ps; q < gd.pk; ++q) {
-            const double2 a = sA[q];
-            const uint32_t zl = uint32_t(__builtin_amdgcn_readfirstlane(int(sZ[q])));
-            const uint32_t flip =
-                uint32_t(__popc(tid & zl) + __popc(uint32_t(k) & (zl >> kApplyThreadBits))) << 31;
-            dr += flip_sign(a.x, flip);
-            if (cplx) di += flip_sign(a.y, flip);
-        }
-        const cT xb = xg[k];
+        const cT xb = xg[k ^ int(kSwap)];
         if (!kMixed) {
-            ar[k] = fma(dr, xb.x, ar[k]);
-            ai[k] = fma(dr, xb.y, ai[k]);
+            ar[k] = fma(double(dre[k]), xb.x, ar[k]);
+            ai[k] = fma(double(dre[k]), xb.y, ai[k]);
             if (cplx) {
-                ar[k] = fma(-di, xb.y, ar[k]);
-                ai[k] = fma(di, xb.x, ai[k]);
+                ar[k] = fma(-double(dim[k]), xb.y, ar[k]);
+                ai[k] = fma(double(dim[k]), xb.x, ai[k]);
             }
         } else {
-            const float fr = float(dr);
-            br[k] = fmaf(fr, xb.x, br[k]);
-            bi[k] = fmaf(fr, xb.y, bi[k]);
+            br[k] = fmaf(float(dre[k]), xb.x, br[k]);
+            bi[k] = fmaf(float(dre[k]), xb.y, bi[k]);
             if (cplx) {
-                const float fi = float(di);
-                br[k] = fmaf(-fi, xb.y, br[k]);
-                bi[k] = fmaf(fi, xb.x, bi[k]);
+                br[k] = fmaf(-float(dim[k]), xb.y, br[k]);
+                bi[k] = fmaf(float(dim[k]), xb.x, bi[k]);
             }
         }
     }
@@ -514,9 +514,9 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_tiled(
         chunk_tables<kNarrow, R>(ch, tid, base, zero_row, grp, ug, pzl, ptptr, ptz, ptc, sA, sZ, sU);
         // one loop per group class, each with a single straight path, in
         // blocks of at most kCompBlock groups flushed after each block
-        auto run_class = [&](auto far_t, auto rowvar_t, auto cplx_t, uint32_t g0, uint32_t g1) {
+        auto run_class = [&](auto far_t, auto rowvar_t, auto cplx_t, auto swap_t, uint32_t g0, uint32_t g1) {
             constexpr bool kFar = decltype(far_t)::value, kRowvar = decltype(rowvar_t)::value;
-            constexpr bool kCplx = decltype(cplx_t)::value;
+            constexpr bool kCplx = decltype(cplx_t)::value, kSwap = decltype(swap_t)::value;
             for (uint32_t gb = g0; gb < g1;) {
                 // up to the next flush point, counted over all classes
                 const uint32_t room = uint32_t(kCompBlock) - since_flush;
@@ -527,9 +527,11 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_tiled(
                     // so that no later scalar-load wait also waits for them
                     asm volatile("" ::"s"(gd.p0), "s"(gd.ps), "s"(gd.pk), "s"(gd.p1), "s"(gd.u),
                                  "s"(gd.flags));
-                    gather<kFar, kFull, kPairs, R>(gd.m, tid, tile_rows, base, xt, xv, xg);
-                    accumulate<kMixed, kFull, kRowvar, kCplx, R>(gd, tid, tile_rows, zero_row, sA, sZ,
-                                                                 sU, xg, ar, ai, br, bi);
+                    const auto fetch = [&]() {
+                        gather<kFar, kFull, kPairs, R>(gd.m, tid, tile_rows, base, xt, xv, xg);
+                    };
+                    accumulate<kMixed, kFull, kRowvar, kCplx, kSwap, R>(gd, tid, tile_rows, zero_row, sA, sZ,
+                                                                        sU, xg, fetch, ar, ai, br, bi);
                 }
                 since_flush += ge - gb;
                 gb = ge;
@@ -542,14 +544,25 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_tiled(
         };
         using F = std::false_type;
         using T = std::true_type;
-        run_class(F{}, F{}, F{}, ch.cg[0], ch.cg[1]);
-        run_class(T{}, F{}, F{}, ch.cg[1], ch.cg[2]);
-        run_class(F{}, T{}, F{}, ch.cg[2], ch.cg[3]);
-        run_class(T{}, T{}, F{}, ch.cg[3], ch.cg[4]);
-        run_class(F{}, F{}, T{}, ch.cg[4], ch.cg[5]);
-        run_class(T{}, F{}, T{}, ch.cg[5], ch.cg[6]);
-        run_class(F{}, T{}, T{}, ch.cg[6], ch.cg[7]);
-        run_class(T{}, T{}, T{}, ch.cg[7], ch.cg[8]);
+        run_class(F{}, F{}, F{}, F{}, ch.cg[0], ch.cg[1]);
+        run_class(T{}, F{}, F{}, F{}, ch.cg[1], ch.cg[2]);
+        run_class(F{}, T{}, F{}, F{}, ch.cg[2], ch.cg[3]);
+        run_class(T{}, T{}, F{}, F{}, ch.cg[3], ch.cg[4]);
+        run_class(F{}, F{}, T{}, F{}, ch.cg[4], ch.cg[5]);
+        run_class(T{}, F{}, T{}, F{}, ch.cg[5], ch.cg[6]);
+        run_class(F{}, T{}, T{}, F{}, ch.cg[6], ch.cg[7]);
+        run_class(T{}, T{}, T{}, F{}, ch.cg[7], ch.cg[8]);
+        if constexpr (kPairs) {
+            // masks with row bit 0: each row's partner is the other half of its pair
+            run_class(F{}, F{}, F{}, T{}, ch.cg[8], ch.cg[9]);
+            run_class(T{}, F{}, F{}, T{}, ch.cg[9], ch.cg[10]);
+            run_class(F{}, T{}, F{}, T{}, ch.cg[10], ch.cg[11]);
+            run_class(T{}, T{}, F{}, T{}, ch.cg[11], ch.cg[12]);
+            run_class(F{}, F{}, T{}, T{}, ch.cg[12], ch.cg[13]);
+            run_class(T{}, F{}, T{}, T{}, ch.cg[13], ch.cg[14]);
+            run_class(F{}, T{}, T{}, T{}, ch.cg[14], ch.cg[15]);
+            run_class(T{}, T{}, T{}, T{}, ch.cg[15], ch.cg[16]);
+        }
     }
 
     if (kMixed && since_flush) flush();
@@ -716,9 +729,9 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_staged(
                     const ApplyGroup gd = grp[g];
                     asm volatile("" ::"s"(gd.p0), "s"(gd.ps), "s"(gd.pk), "s"(gd.p1), "s"(gd.u),
                                  "s"(gd.flags));
-                    gather<false, true, false, R>(gd.m, tid, tile_rows, base, xt, xv, xg);
-                    accumulate<kMixed, true, kRowvar, kCplx, R>(gd, tid, tile_rows, zero_row, sA, sZ,
-                                                                sU, xg, ar, ai, br, bi);
+                    const auto fetch = [&]() { gather<false, true, false, R>(gd.m, tid, tile_rows, base, xt, xv, xg); };
+                    accumulate<kMixed, true, kRowvar, kCplx, false, R>(gd, tid, tile_rows, zero_row, sA, sZ,
+                                                                       sU, xg, fetch, ar, ai, br, bi);
                 }
                 since_flush += ge - gb;
                 gb = ge;

@@ -70,10 +70,12 @@ constexpr uint32_t kGroupReal = 1;  // every coefficient of the group is real
 // Groups whose patterns [p0, p1) and table rows [u0, u1) are staged in LDS
 // together.  Inside a chunk the groups are ordered by class -- bit 0: the mask
 // reaches outside the tile ("far"), bit 1: D varies by row (row-bit or
-// straddling patterns), bit 2: complex coefficients -- and class c holds the
-// groups [cg[c], cg[c+1]).  For the partner-staged kernel the groups are
-// ordered by partner tile instead, in runs [r0, r1) (ApplyRun).
-constexpr int kApplyClasses = 8;
+// straddling patterns), bit 2: complex coefficients, bit 3 (paired-row layout
+// only): row bit 0 of the mask set, so that each row's partner is the other
+// half of the gathered pair -- and class c holds the groups [cg[c], cg[c+1]).
+// For the partner-staged kernel the groups are ordered by partner tile
+// instead, in runs [r0, r1) (ApplyRun).
+constexpr int kApplyClasses = 16;
 struct ApplyChunk {
     uint32_t cg[kApplyClasses + 1];
     uint32_t p0, p1, u0, u1;
