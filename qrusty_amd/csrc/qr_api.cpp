@@ -100,7 +100,7 @@ static void build_rank_lists(const std::vector<uint64_t> &gx, std::vector<uint32
 }
 
 void free_tables(DeviceTables &t) {
-    void *ptrs[] = {t.gx, t.gterm, t.tz, t.tc, t.rl_ptr, t.rl, t.gkind, t.heavy, t.leaf_heavy};
+    void *ptrs[] = {t.gx, t.gterm, t.tz, t.tc, t.rl_ptr, t.rl, t.gkind, t.heavy, t.leaf_heavy, t.ttz, t.ttc};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &pt : t.pat) {
@@ -250,6 +250,16 @@ static int upload(T **dst, const std::vector<T> &src) {
     return QR_OK;
 }
 
+int upload_tail(const qr_op *op, DeviceTables &t) {
+    if (!op->ntail) return QR_OK;
+    std::vector<double2> c2(op->ttz.size());
+    for (size_t i = 0; i < c2.size(); ++i) c2[i] = make_double2(op->ttc[2 * i], op->ttc[2 * i + 1]);
+    int rc;
+    if ((rc = upload(&t.ttz, op->ttz)) || (rc = upload(&t.ttc, c2))) return rc;
+    t.ntail = op->ntail;
+    return QR_OK;
+}
+
 int device_tables(const qr_op *op, DeviceTables **out, bool noalloc) {
     int dev = 0;
     QR_HIP_CHECK(hipGetDevice(&dev));
@@ -268,7 +278,8 @@ int device_tables(const qr_op *op, DeviceTables **out, bool noalloc) {
             (rc = upload(&t.rl_ptr, op->rl_ptr)) || (rc = upload(&t.rl, op->rl)) ||
             (rc = upload(&t.gkind, op->gkind)) ||
             (!op->heavy.empty() && (rc = upload(&t.heavy, op->heavy))) ||
-            (!op->leaf_heavy.empty() && (rc = upload(&t.leaf_heavy, op->leaf_heavy)))) {
+            (!op->leaf_heavy.empty() && (rc = upload(&t.leaf_heavy, op->leaf_heavy))) ||
+            (rc = upload_tail(op, t))) {
             free_tables(t);
             return rc;
         }
@@ -357,6 +368,30 @@ void finish_op(qr_op *op) {
             for (uint32_t g : op->heavy)
                 if (g >= lf.s0 && g < lf.s1) op->leaf_heavy.push_back(g - lf.s0);
             lf.hcnt = uint32_t(op->leaf_heavy.size()) - lf.hoff;
+        }
+    }
+    // tail table: operators with heavy groups, whose other groups also run
+    // past two terms (the molecular Hamiltonians: 4 and 6)
+    if (!op->heavy.empty()) {
+        uint32_t nt = 0;
+        for (uint64_t g = 0; g < op->G; ++g) {
+            const uint32_t n = op->gterm[g + 1] - op->gterm[g];
+            if (n > 2 && !std::binary_search(op->heavy.begin(), op->heavy.end(), uint32_t(g)))
+                nt = std::max(nt, n - 2);
+        }
+        nt = std::min(nt, kMaxTail);
+        while (nt && uint64_t(nt) * op->G > kMaxTailSlots) --nt;
+        op->ntail = nt;
+        op->ttz.assign(uint64_t(nt) * op->G, 0);
+        op->ttc.assign(2 * uint64_t(nt) * op->G, 0.0);
+        for (uint64_t g = 0; g < op->G; ++g) {
+            const uint32_t t0 = op->gterm[g], n = op->gterm[g + 1] - t0;
+            for (uint32_t u = 0; u < nt && u + 2 < n; ++u) {
+                const uint64_t slot = uint64_t(u) * op->G + g;
+                op->ttz[slot] = op->tz[t0 + 2 + u];
+                op->ttc[2 * slot] = op->tc[2 * (t0 + 2 + u)];
+                op->ttc[2 * slot + 1] = op->tc[2 * (t0 + 2 + u) + 1];
+            }
         }
     }
 }

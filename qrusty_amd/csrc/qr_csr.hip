@@ -85,7 +85,8 @@ int launch_csr_build(const qr_op *op, const DeviceTables *t, uint64_t row_begin,
             TileOp top{t->gx + lf.s0, t->gterm + lf.s0, t->tz, t->tc, t->rl_ptr + lf.s0, t->rl,
                        d_indices, reinterpret_cast<double2 *>(d_data), nullptr, nullptr, nullptr,
                        row_begin, row_end, g, jw, nch, dbuf,
-                       nh ? t->leaf_heavy + lf.hoff : nullptr, nh, lf.s0, lf.depth, G};
+                       nh ? t->leaf_heavy + lf.hoff : nullptr, nh, lf.s0, lf.depth, G,
+                       t->ntail ? t->ttz + lf.s0 : nullptr, t->ntail ? t->ttc + lf.s0 : nullptr, t->ntail};
             // (wide masks, and heavy groups, do not fit 64 VGPRs without
             // spills: one block per CU)
             auto kst = nh ? (narrow ? (nch <= 1 ? k_csr_build_staged<true, 1, kTileSubHeavy>
@@ -119,7 +120,7 @@ int launch_csr_build(const qr_op *op, const DeviceTables *t, uint64_t row_begin,
     const size_t shmem = stage_bytes(G, jw, &dbuf) + heavy_bytes(nh, jw);
     TileOp top{t->gx, t->gterm, t->tz, t->tc, t->rl_ptr, t->rl, d_indices,
                reinterpret_cast<double2 *>(d_data), nullptr, nullptr, nullptr,
-               row_begin, row_end, G, jw, nch, dbuf, t->heavy, nh, 0, 0, G};
+               row_begin, row_end, G, jw, nch, dbuf, t->heavy, nh, 0, 0, G, t->ttz, t->ttc, t->ntail};
     auto pick = [&](auto narrow_t, auto heavy_t) {
         constexpr bool kN = decltype(narrow_t)::value;
         constexpr int kH = decltype(heavy_t)::value ? kTileHeavy : kTilePlain;

@@ -487,7 +487,8 @@ int group_terms_device(const char *const *labels, const double *coeffs, uint64_t
         return QR_OK;
     };
     if ((!op->heavy.empty() && (rc = put(&t.heavy, op->heavy))) ||
-        (!op->leaf_heavy.empty() && (rc = put(&t.leaf_heavy, op->leaf_heavy)))) {
+        (!op->leaf_heavy.empty() && (rc = put(&t.leaf_heavy, op->leaf_heavy))) ||
+        (rc = upload_tail(op, t))) {
         free_tables(t);
         delete op;
         return rc;

@@ -42,6 +42,11 @@ constexpr int kMaxDevices = 16;
 // next to groups of at most 6 (as many as the tile's LDS holds are folded
 // that way, heavy_fit in qr_tile.h)
 constexpr uint32_t kMaxHeavy = 128;
+// Tail table of an operator with heavy groups: terms 2, 3, ... of every group
+// transposed, slot u * G + g = term 2 + u of group g, for u < ntail <=
+// kMaxTail and at most kMaxTailSlots slots (qr_op::ttz, ttc)
+constexpr uint32_t kMaxTail = 16;
+constexpr uint64_t kMaxTailSlots = uint64_t(1) << 24;
 constexpr uint32_t kSubMaxG = 2048;  // groups of one CSR-build sub-operator
 // CSR matvecs load the CSR stream nontemporally from this many entries per
 // row on (measured crossover; qr_csr.hip)
@@ -107,6 +112,9 @@ struct DeviceTables {
     uint8_t *gkind = nullptr;    // [G]   0 complex, 1 real, 2 imaginary coefficients
     uint32_t *heavy = nullptr;   // [nheavy] heavy groups (qr_op::heavy)
     uint32_t nheavy = 0;
+    uint64_t *ttz = nullptr;     // [ntail * G] tail table (qr_op::ttz, ttc)
+    double2 *ttc = nullptr;
+    uint32_t ntail = 0;
     uint32_t *leaf_heavy = nullptr;  // qr_op::leaf_heavy
     // matrix-free apply tables, one set per precision (tile size) and form
     // (slot = precision + 2 * staged)
@@ -182,6 +190,13 @@ struct qr_op {
     };
     std::vector<SubLeaf> leaves;
     std::vector<uint32_t> leaf_heavy;  // heavy groups of each leaf, leaf-local
+    // operators with heavy groups: the terms past each group's second,
+    // transposed (slot u * G + g holds term gterm[g] + 2 + u, zero where the
+    // group is shorter), so that the lanes of a batch -- one group each --
+    // read one contiguous run per term position instead of one line apiece
+    uint32_t ntail = 0;
+    std::vector<uint64_t> ttz;
+    std::vector<double> ttc;  // 2 * ntail * G
     // device state
     mutable std::mutex mu;
     mutable qr::DeviceTables dev[qr::kMaxDevices];
@@ -195,8 +210,11 @@ namespace qr {
 int group_terms_device(const char *const *labels, const double *coeffs, uint64_t T, int32_t conv,
                        hipStream_t s, qr_op **out);
 
-// Sub-operators and heavy groups of an op whose grouped tables are set.
+// Sub-operators, heavy groups and tail table of an op whose grouped tables
+// are set.
 void finish_op(qr_op *op);
+// Uploads the op's tail table (if it has one) into t.
+int upload_tail(const qr_op *op, DeviceTables &t);
 // Frees one device's tables of an op.
 void free_tables(DeviceTables &t);
 

@@ -74,6 +74,11 @@ struct TileOp {
     // gbase; rows are gstride entries apart in the outputs.  An operator
     // built whole has gbase = nanc = 0 and gstride = G.
     uint32_t gbase, nanc, gstride;
+    // tail table (qr_op::ttz, ttc), pointing at group gbase: term 2 + u of
+    // group j at u * gstride + j, for u < ntail; ntail = 0: none
+    const uint64_t *ttz;
+    const double2 *ttc;
+    uint32_t ntail;
 };
 
 // The arrays of a TileOp seen through pointers typed as global memory.  A
@@ -96,6 +101,8 @@ struct TileMem {
     gptr<const dvec2> x;
     gptr<dvec2> y;
     gptr<const uint32_t> heavy;
+    gptr<const uint64_t> ttz;
+    gptr<const dvec2> ttc;
 };
 __device__ __forceinline__ TileMem tile_mem(const TileOp &op) {
     return {as_global(op.gx),
@@ -109,7 +116,9 @@ __device__ __forceinline__ TileMem tile_mem(const TileOp &op) {
             as_global(op.indptr),
             as_global(reinterpret_cast<const dvec2 *>(op.x)),
             as_global(reinterpret_cast<dvec2 *>(op.y)),
-            as_global(op.heavy)};
+            as_global(op.heavy),
+            as_global(op.ttz),
+            as_global(reinterpret_cast<const dvec2 *>(op.ttc))};
 }
 __device__ __forceinline__ RankEntry ld_rank(gptr<const RankEntry> p) {
     RankEntry e;
@@ -152,16 +161,16 @@ __host__ __device__ inline uint32_t heavy_fit(uint32_t nheavy, uint32_t jw_log2)
 }
 
 // Term-order fold of group j in row r: (-1)^popcount(r & z_t) c'_t, the
-// first value, then + (accel.rs:191-208).  Terms are loaded four at a time,
-// all four loads issued before the first add; the adds stay in term order
-// (eight at a time cost the heavy-group instance its second block per CU).
-// Measured on the 24-qubit Heisenberg build: 2.68 -> 2.58 ms.
-template <bool kNarrow>
+// first value, then + (accel.rs:191-208).  Terms are loaded kF at a time,
+// all kF loads issued before the first add; the adds stay in term order.
+// Four at a time measured 2.68 -> 2.58 ms on the 24-qubit Heisenberg build;
+// eight cost that (one-chunk) instance its second block per CU, so eight are
+// used only by the two-chunk instances, which run one block per CU anyway.
+template <bool kNarrow, uint32_t kF>
 __device__ __forceinline__ double2 fold_group(const TileMem &mem, uint32_t j, uint64_t r) {
     const uint32_t t0 = mem.gterm[j], t1 = mem.gterm[j + 1];
     double2 v = signed_coeff(ld2(mem.tc + t0), parity_and<kNarrow>(r, mem.tz[t0]));
     uint32_t t = t0 + 1;
-    constexpr uint32_t kF = 4;
     for (; t + kF <= t1; t += kF) {
         double2 ct[kF];
 #pragma unroll
@@ -321,7 +330,7 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
         for (uint32_t i = tid; i < op.nheavy * tr; i += kStageThreads) {
             const uint64_t r = tile_base + (i & (tr - 1));
             if (r >= row_begin && r < row_end)
-                hv[i] = fold_group<kNarrow>(mem, mem.heavy[i / tr], r);
+                hv[i] = fold_group<kNarrow, kChunks >= 2 ? 8u : 4u>(mem, mem.heavy[i / tr], r);
         }
         __syncthreads();
     }
@@ -364,8 +373,20 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
                                                         parity_and<kNarrow>(r, mem.tz[tf + 1]));
                 v.x = v.x + e.x;
                 v.y = v.y + e.y;
-                for (uint32_t t = tf + 2; t < t1[c]; ++t) {
-                    const double2 ct = signed_coeff(ld2(mem.tc + t), parity_and<kNarrow>(r, mem.tz[t]));
+                // the remaining terms, in term order: from the transposed tail
+                // table while it has them -- the lanes, one group each, then
+                // read one contiguous run per position (u counts iterations,
+                // so the choice is the same for every active lane) -- else
+                // from the group's own term range.  Four loads in flight were
+                // slower than this serial loop, from either table.
+                for (uint32_t t = tf + 2, u = 0; t < t1[c]; ++t, ++u) {
+                    double2 ct;
+                    if (u < op.ntail) {
+                        const uint32_t slot = u * op.gstride + j;
+                        ct = signed_coeff(ld2(mem.ttc + slot), parity_and<kNarrow>(r, mem.ttz[slot]));
+                    } else {
+                        ct = signed_coeff(ld2(mem.tc + t), parity_and<kNarrow>(r, mem.tz[t]));
+                    }
                     v.x = v.x + ct.x;
                     v.y = v.y + ct.y;
                 }

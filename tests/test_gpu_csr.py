@@ -400,6 +400,48 @@ def test_sub_operators_with_heavy_group(n):
             assert np.array_equal(got_dv.view(np.uint64), vals.view(np.uint64)), f"row {r}"
 
 
+def _grouped_labels(rng, n, sizes):
+    """Labels in groups: one random x mask per entry of `sizes`, with that many
+    terms of random z masks (Y where x and z meet)."""
+    masks = rng.choice(1 << n, size=len(sizes), replace=False)
+    chars = {(0, 0): "I", (0, 1): "Z", (1, 0): "X", (1, 1): "Y"}
+    labels = []
+    for xm, k in zip(masks, sizes):
+        for zm in rng.integers(0, 1 << n, size=k):
+            labels.append("".join(chars[((int(xm) >> q) & 1, (int(zm) >> q) & 1)]
+                                  for q in reversed(range(n))))
+    return labels
+
+
+@pytest.mark.parametrize("case", ["single", "sub"])
+def test_tail_table_groups_bit_exact(case):
+    """Operators with heavy groups fold the terms past each group's second
+    from a transposed tail table (at most 16 positions).  Groups of 6 terms
+    (the molecular shape), groups of 20 (tails past the table's 16
+    positions, read from the group's own terms), and more heavy groups than
+    one tile's LDS holds (the ones left out run the batch loop, past the
+    table) -- single operator (two-chunk heavy instance) and sub-operators
+    (G > 2048, the table offset per subtree).  All rows bit-exact."""
+    rng = np.random.default_rng(4711 if case == "single" else 4712)
+    if case == "single":
+        n, sizes = 11, [6] * 150 + [20] * 3 + [60] * 20
+    else:
+        n, sizes = 13, [4] * 2100 + [6] * 40 + [60] * 10
+    labels = _grouped_labels(rng, n, sizes)
+    order = rng.permutation(len(labels))
+    labels = [labels[i] for i in order]
+    coeffs = rng.standard_normal(len(labels)) + 1j * rng.standard_normal(len(labels))
+    p = orc.Params(labels, coeffs)
+    G = len(np.unique(p.x))
+    print(f"[gpu] tail table {case} n={n} T={len(labels)} G={G}", flush=True)
+    assert G == len(sizes) and (G > 2048) == (case == "sub")
+    ip, ind, dat = orc.build_csr(p)
+    shape, data, indices, indptr = _export(_op(labels, coeffs).to_matrix_mode("Hip"))
+    assert indptr.tobytes() == ip.tobytes()
+    assert indices.tobytes() == ind.tobytes()
+    assert data.view(np.uint64).tobytes() == dat.view(np.uint64).tobytes()
+
+
 @pytest.mark.parametrize("n,T,seed", [(15, 140, 1), (14, 300, 2), (15, 520, 3), (13, 64, 4),
                                       (14, 1100, 5)])
 def test_mid_size_ops_bit_exact(n, T, seed):
