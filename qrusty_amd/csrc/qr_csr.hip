@@ -34,11 +34,23 @@
 namespace qr {
 
 // LDS-staged build: one block per tile of the shared tile builder
-// (qr_tile.h), without the fused matvec.
+// (qr_tile.h), without the fused matvec; the two-chunk instances for
+// operators with heavy groups run 2^op.fold_log2 consecutive tiles per block,
+// folding the heavy groups once for all of them.
 template <bool kNarrow, int kChunks, int kMode>
 __global__ __launch_bounds__(kStageThreads) void k_csr_build_staged(TileOp op) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    build_tile_staged<kNarrow, kChunks, false, kMode>(op, blockIdx.x, smem_raw);
+    if constexpr ((kMode == kTileHeavy || kMode == kTileSubHeavy) && kChunks >= 2) {
+        const uint32_t tile_log2 = 10u - op.jw_log2 + 4u;
+        const uint32_t k = 1u << op.fold_log2;
+        for (uint32_t s = 0; s < k; ++s) {
+            const uint64_t tile = uint64_t(blockIdx.x) * k + s;
+            if ((((op.row_begin >> tile_log2) + tile) << tile_log2) >= op.row_end) break;
+            build_tile_staged<kNarrow, kChunks, false, kMode>(op, tile, smem_raw, s);
+        }
+    } else {
+        build_tile_staged<kNarrow, kChunks, false, kMode>(op, blockIdx.x, smem_raw);
+    }
 }
 
 // Sub-operators of at most 32 qubits (at most 2048 groups, a 48 KB stage):
@@ -81,12 +93,14 @@ int launch_csr_build(const qr_op *op, const DeviceTables *t, uint64_t row_begin,
             const uint32_t g = lf.s1 - lf.s0, nch = (g + 1023) >> 10;
             uint32_t dbuf = 0;
             const uint32_t nh = heavy_fit(lf.hcnt, jw);
-            const size_t shmem = stage_bytes(g, jw, &dbuf) + heavy_bytes(nh, jw);
+            const size_t sb = stage_bytes(g, jw, &dbuf);
+            const uint32_t fl = fold_tiles_log2(nch, sb, heavy_bytes(nh, jw));
+            const size_t shmem = sb + (heavy_bytes(nh, jw) << fl);
             TileOp top{t->gx + lf.s0, t->gterm + lf.s0, t->tz, t->tc, t->rl_ptr + lf.s0, t->rl,
                        d_indices, reinterpret_cast<double2 *>(d_data), nullptr, nullptr, nullptr,
                        row_begin, row_end, g, jw, nch, dbuf,
                        nh ? t->leaf_heavy + lf.hoff : nullptr, nh, lf.s0, lf.depth, G,
-                       t->ntail ? t->ttz + lf.s0 : nullptr, t->ntail ? t->ttc + lf.s0 : nullptr, t->ntail};
+                       t->ntail ? t->ttz + lf.s0 : nullptr, t->ntail ? t->ttc + lf.s0 : nullptr, t->ntail, fl};
             // (wide masks, and heavy groups, do not fit 64 VGPRs without
             // spills: one block per CU)
             auto kst = nh ? (narrow ? (nch <= 1 ? k_csr_build_staged<true, 1, kTileSubHeavy>
@@ -99,7 +113,7 @@ int launch_csr_build(const qr_op *op, const DeviceTables *t, uint64_t row_begin,
                                                 : k_csr_build_staged<false, 2, kTileSub>));
             QR_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(kst),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem)));
-            hipLaunchKernelGGL(kst, dim3(unsigned(nt)), dim3(kStageThreads), shmem, s, top);
+            hipLaunchKernelGGL(kst, dim3(unsigned(((nt - 1) >> fl) + 1)), dim3(kStageThreads), shmem, s, top);
             QR_HIP_CHECK(hipGetLastError());
         }
         return QR_OK;
@@ -117,10 +131,12 @@ int launch_csr_build(const qr_op *op, const DeviceTables *t, uint64_t row_begin,
     if (nt > 0x7FFFFFFFull) return fail(QR_ERR_SIZE, "CSR build grid too large");
     uint32_t dbuf = 0;
     const uint32_t nh = heavy_fit(t->nheavy, jw);
-    const size_t shmem = stage_bytes(G, jw, &dbuf) + heavy_bytes(nh, jw);
+    const size_t sb = stage_bytes(G, jw, &dbuf);
+    const uint32_t fl = fold_tiles_log2(nch, sb, heavy_bytes(nh, jw));
+    const size_t shmem = sb + (heavy_bytes(nh, jw) << fl);
     TileOp top{t->gx, t->gterm, t->tz, t->tc, t->rl_ptr, t->rl, d_indices,
                reinterpret_cast<double2 *>(d_data), nullptr, nullptr, nullptr,
-               row_begin, row_end, G, jw, nch, dbuf, t->heavy, nh, 0, 0, G, t->ttz, t->ttc, t->ntail};
+               row_begin, row_end, G, jw, nch, dbuf, t->heavy, nh, 0, 0, G, t->ttz, t->ttc, t->ntail, fl};
     auto pick = [&](auto narrow_t, auto heavy_t) {
         constexpr bool kN = decltype(narrow_t)::value;
         constexpr int kH = decltype(heavy_t)::value ? kTileHeavy : kTilePlain;
@@ -131,7 +147,7 @@ int launch_csr_build(const qr_op *op, const DeviceTables *t, uint64_t row_begin,
     auto kst = narrow ? (nh ? pick(T{}, T{}) : pick(T{}, F{})) : (nh ? pick(F{}, T{}) : pick(F{}, F{}));
     QR_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(kst),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem)));
-    hipLaunchKernelGGL(kst, dim3(unsigned(nt)), dim3(kStageThreads), shmem, s, top);
+    hipLaunchKernelGGL(kst, dim3(unsigned(((nt - 1) >> fl) + 1)), dim3(kStageThreads), shmem, s, top);
     QR_HIP_CHECK(hipGetLastError());
     return QR_OK;
 }

@@ -79,6 +79,9 @@ struct TileOp {
     const uint64_t *ttz;
     const double2 *ttc;
     uint32_t ntail;
+    // heavy-group instances: each block builds 2^fold_log2 consecutive tiles
+    // and folds the heavy groups once for all of their rows
+    uint32_t fold_log2;
 };
 
 // The arrays of a TileOp seen through pointers typed as global memory.  A
@@ -159,6 +162,18 @@ __host__ __device__ inline uint32_t heavy_fit(uint32_t nheavy, uint32_t jw_log2)
     while (nheavy && heavy_bytes(nheavy, jw_log2) > kHeavyMaxBytes) --nheavy;
     return nheavy;
 }
+// Tiles per block (log2: 2, 1 or 0) of a two-chunk heavy-group instance, as
+// many as keep the stage and their heavy values within kFoldBudget bytes of
+// LDS (those instances run one block per CU for their registers).  The
+// one-chunk instances build one tile per block: the tile loop would take
+// them past 64 VGPRs and cost them their second block per CU.
+constexpr size_t kFoldBudget = 150 * 1024;
+__host__ __device__ inline uint32_t fold_tiles_log2(uint32_t nchunks, size_t stage, size_t heavy) {
+    if (nchunks < 2 || heavy == 0) return 0;
+    uint32_t f = 2;
+    while (f && stage + (heavy << f) > kFoldBudget) --f;
+    return f;
+}
 
 // Term-order fold of group j in row r: (-1)^popcount(r & z_t) c'_t, the
 // first value, then + (accel.rs:191-208).  Terms are loaded kF at a time,
@@ -201,9 +216,12 @@ __host__ __device__ inline uint32_t stage_jw_log2(uint32_t G) {
 // Separate instantiations, so that plain operators run the plain loop (its
 // register count decides whether two blocks share a CU).
 constexpr int kTilePlain = 0, kTileHeavy = 1, kTileSub = 2, kTileSubHeavy = 3;
+// `sub` is the tile's place among the 2^op.fold_log2 tiles of its block
+// (heavy-group instances; 0 otherwise): the first folds the heavy groups for
+// all of them.
 template <bool kNarrow, int kChunks, bool kMatvec, int kMode = kTilePlain>
 __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t tile,
-                                                  unsigned char *smem_raw) {
+                                                  unsigned char *smem_raw, uint32_t sub = 0) {
     constexpr bool kHeavy = kMode == kTileHeavy || kMode == kTileSubHeavy;
     constexpr bool kSub = kMode == kTileSub || kMode == kTileSubHeavy;
     // the second term of each group stays in registers when few chunks leave
@@ -230,8 +248,10 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
     // waited for all earlier global stores
     const uint32_t stage_b = batch_pad * 24u;
     double2 *red = reinterpret_cast<double2 *>(smem_raw + (op.dbuf ? 2u : 1u) * stage_b);
-    // heavy-group values of the tile's rows: hv[h << tile_log2 | (r - tile_base)]
+    // heavy-group values of the rows of the block's tiles:
+    // hv[h << (tile_log2 + fold_log2) | sub << tile_log2 | (r - tile_base)]
     double2 *hv = reinterpret_cast<double2 *>(smem_raw + stage_bytes(G, jw_log2, nullptr));
+    const uint32_t fold_log2 = (kHeavy && kChunks >= 2) ? op.fold_log2 : 0u;
 
     // rows of the tile: indptr
     if (op.indptr) {
@@ -323,10 +343,12 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
         }
     }
 
-    if (kHeavy) {
+    if (kHeavy && sub == 0) {
         // heavy groups: one thread per (group, row) folds the whole group, so
-        // no lane of a batch waits on a long term chain
-        const uint32_t tr = uint32_t(kBuildM) << rs_log2;
+        // no lane of a batch waits on a long term chain; the block's first
+        // tile folds for the rows of all its tiles, so that the longest
+        // chain, which sets the time of the pass, is paid once for them
+        const uint32_t tr = uint32_t(kBuildM) << (rs_log2 + fold_log2);
         for (uint32_t i = tid; i < op.nheavy * tr; i += kStageThreads) {
             const uint64_t r = tile_base + (i & (tr - 1));
             if (r >= row_begin && r < row_end)
@@ -366,7 +388,8 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
             double2 v = kLean ? signed_coeff(ld2(mem.tc + tf), parity_and<kNarrow>(r, mem.tz[tf]))
                               : signed_coeff(c0[c], parity_and<kNarrow>(r, z0[c]));
             if (kHeavy && hs[c] >= 0) {
-                v = hv[(uint32_t(hs[c]) << tile_log2) | (uint32_t(m) << rs_log2) | ro];
+                v = hv[(uint32_t(hs[c]) << (tile_log2 + fold_log2)) | (sub << tile_log2) |
+                       (uint32_t(m) << rs_log2) | ro];
             } else if (t1[c] > tf + 1) {
                 const double2 e = kHold2 ? signed_coeff(c1[c], parity_and<kNarrow>(r, z1[c]))
                                          : signed_coeff(ld2(mem.tc + tf + 1),
