@@ -44,9 +44,10 @@ constexpr int kMaxDevices = 16;
 constexpr uint32_t kMaxHeavy = 128;
 // Tail table of an operator with heavy groups: terms 2, 3, ... of every group
 // transposed, slot u * G + g = term 2 + u of group g, for u < ntail <=
-// kMaxTail and at most kMaxTailSlots slots (qr_op::ttz, ttc)
+// kMaxTail and at most kMaxTailSlots slots (24 B each: at most 96 MiB per
+// device; the builder reads the terms past the table from the grouped table)
 constexpr uint32_t kMaxTail = 16;
-constexpr uint64_t kMaxTailSlots = uint64_t(1) << 24;
+constexpr uint64_t kMaxTailSlots = uint64_t(1) << 22;
 constexpr uint32_t kSubMaxG = 2048;  // groups of one CSR-build sub-operator
 // CSR matvecs load the CSR stream nontemporally from this many entries per
 // row on (measured crossover; qr_csr.hip)
