@@ -442,6 +442,39 @@ def test_tail_table_groups_bit_exact(case):
     assert data.view(np.uint64).tobytes() == dat.view(np.uint64).tobytes()
 
 
+def test_heavy_fold_row_windows_bit_exact():
+    """qr_csr_build_rows over row windows that start and end inside a tile,
+    for an operator whose two-chunk heavy-group instance builds several
+    tiles per block with one heavy fold: each window's rows equal the
+    oracle's bit for bit (first tile unaligned, last block partial, rows past
+    the window inside a fold)."""
+    import ctypes
+    from qrusty_amd import _lib as L
+    rng = np.random.default_rng(4713)
+    n, sizes = 11, [6] * 150 + [20] * 3 + [60] * 20
+    labels = _grouped_labels(rng, n, sizes)
+    coeffs = rng.standard_normal(len(labels)) + 1j * rng.standard_normal(len(labels))
+    p = orc.Params(labels, coeffs)
+    G = len(sizes)
+    ip, ind, dat = orc.build_csr(p)
+    ind = ind.reshape(-1, G)
+    dat = dat.view(np.uint64).reshape(-1, 2 * G)
+    h = _op(labels, coeffs)._handle(0)
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    N = 1 << n
+    for r0, r1 in [(5, 300), (129, 130), (1000, N - 3), (0, N), (N - 77, N)]:
+        print(f"[gpu] heavy fold rows [{r0}, {r1})", flush=True)
+        W = r1 - r0
+        ix = torch.empty(W * G, dtype=torch.int64, device="cuda")
+        dv = torch.empty(W * G, dtype=torch.complex128, device="cuda")
+        L.check(L.lib.qr_csr_build_rows(h.h, r0, r1, ctypes.c_void_p(ix.data_ptr()),
+                                        ctypes.c_void_p(dv.data_ptr()), st))
+        got_ix = ix.cpu().numpy().view(np.uint64).reshape(W, G)
+        got_dv = dv.cpu().numpy().view(np.uint64).reshape(W, 2 * G)
+        assert np.array_equal(got_ix, ind[r0:r1]), f"indices [{r0}, {r1})"
+        assert np.array_equal(got_dv, dat[r0:r1]), f"values [{r0}, {r1})"
+
+
 @pytest.mark.parametrize("n,T,seed", [(15, 140, 1), (14, 300, 2), (15, 520, 3), (13, 64, 4),
                                       (14, 1100, 5)])
 def test_mid_size_ops_bit_exact(n, T, seed):
