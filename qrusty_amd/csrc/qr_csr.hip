@@ -34,13 +34,13 @@
 namespace qr {
 
 // LDS-staged build: one block per tile of the shared tile builder
-// (qr_tile.h), without the fused matvec; the two-chunk instances for
-// operators with heavy groups run 2^op.fold_log2 consecutive tiles per block,
-// folding the heavy groups once for all of them.
+// (qr_tile.h), without the fused matvec; the heavy-group instances of
+// fold_tiles run 2^op.fold_log2 consecutive tiles per block, folding the
+// heavy groups once for all of them.
 template <bool kNarrow, int kChunks, int kMode>
 __global__ __launch_bounds__(kStageThreads) void k_csr_build_staged(TileOp op) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    if constexpr ((kMode == kTileHeavy || kMode == kTileSubHeavy) && kChunks >= 2) {
+    if constexpr (fold_tiles<kChunks, kMode>()) {
         const uint32_t tile_log2 = 10u - op.jw_log2 + 4u;
         const uint32_t k = 1u << op.fold_log2;
         for (uint32_t s = 0; s < k; ++s) {
@@ -94,7 +94,7 @@ int launch_csr_build(const qr_op *op, const DeviceTables *t, uint64_t row_begin,
             uint32_t dbuf = 0;
             const uint32_t nh = heavy_fit(lf.hcnt, jw);
             const size_t sb = stage_bytes(g, jw, &dbuf);
-            const uint32_t fl = fold_tiles_log2(nch, sb, heavy_bytes(nh, jw));
+            const uint32_t fl = fold_tiles_log2(true, sb, heavy_bytes(nh, jw));
             const size_t shmem = sb + (heavy_bytes(nh, jw) << fl);
             TileOp top{t->gx + lf.s0, t->gterm + lf.s0, t->tz, t->tc, t->rl_ptr + lf.s0, t->rl,
                        d_indices, reinterpret_cast<double2 *>(d_data), nullptr, nullptr, nullptr,
@@ -132,7 +132,7 @@ int launch_csr_build(const qr_op *op, const DeviceTables *t, uint64_t row_begin,
     uint32_t dbuf = 0;
     const uint32_t nh = heavy_fit(t->nheavy, jw);
     const size_t sb = stage_bytes(G, jw, &dbuf);
-    const uint32_t fl = fold_tiles_log2(nch, sb, heavy_bytes(nh, jw));
+    const uint32_t fl = fold_tiles_log2(nch >= 2, sb, heavy_bytes(nh, jw));
     const size_t shmem = sb + (heavy_bytes(nh, jw) << fl);
     TileOp top{t->gx, t->gterm, t->tz, t->tc, t->rl_ptr, t->rl, d_indices,
                reinterpret_cast<double2 *>(d_data), nullptr, nullptr, nullptr,

@@ -162,14 +162,13 @@ __host__ __device__ inline uint32_t heavy_fit(uint32_t nheavy, uint32_t jw_log2)
     while (nheavy && heavy_bytes(nheavy, jw_log2) > kHeavyMaxBytes) --nheavy;
     return nheavy;
 }
-// Tiles per block (log2: 2, 1 or 0) of a two-chunk heavy-group instance, as
-// many as keep the stage and their heavy values within kFoldBudget bytes of
-// LDS (those instances run one block per CU for their registers).  The
-// one-chunk instances build one tile per block: the tile loop would take
-// them past 64 VGPRs and cost them their second block per CU.
+// Tiles per block (log2: 2, 1 or 0) of a heavy-group instance that builds
+// several (fold_tiles), as many as keep the stage and their heavy values
+// within kFoldBudget bytes of LDS (those instances run one block per CU for
+// their registers).
 constexpr size_t kFoldBudget = 150 * 1024;
-__host__ __device__ inline uint32_t fold_tiles_log2(uint32_t nchunks, size_t stage, size_t heavy) {
-    if (nchunks < 2 || heavy == 0) return 0;
+__host__ __device__ inline uint32_t fold_tiles_log2(bool several, size_t stage, size_t heavy) {
+    if (!several || heavy == 0) return 0;
     uint32_t f = 2;
     while (f && stage + (heavy << f) > kFoldBudget) --f;
     return f;
@@ -216,6 +215,17 @@ __host__ __device__ inline uint32_t stage_jw_log2(uint32_t G) {
 // Separate instantiations, so that plain operators run the plain loop (its
 // register count decides whether two blocks share a CU).
 constexpr int kTilePlain = 0, kTileHeavy = 1, kTileSub = 2, kTileSubHeavy = 3;
+// Heavy-group instances that build several tiles per block, folding the heavy
+// groups once for all of them: the two-chunk ones (one block per CU anyway)
+// and the one-chunk sub-operators, which the tile loop takes from 60 to 78
+// VGPRs and one block per CU.  Measured (ms): H10 16.75 -> 16.45 and H11
+// 13.92 -> 13.48 with the one-chunk sub-operators in; the whole one-chunk
+// operators (the 24-qubit Heisenberg build) stay at one tile per block
+// (2.25 ms against 3.03 with it).
+template <int kChunks, int kMode>
+__host__ __device__ constexpr bool fold_tiles() {
+    return (kMode == kTileHeavy && kChunks >= 2) || kMode == kTileSubHeavy;
+}
 // `sub` is the tile's place among the 2^op.fold_log2 tiles of its block
 // (heavy-group instances; 0 otherwise): the first folds the heavy groups for
 // all of them.
@@ -251,7 +261,7 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
     // heavy-group values of the rows of the block's tiles:
     // hv[h << (tile_log2 + fold_log2) | sub << tile_log2 | (r - tile_base)]
     double2 *hv = reinterpret_cast<double2 *>(smem_raw + stage_bytes(G, jw_log2, nullptr));
-    const uint32_t fold_log2 = (kHeavy && kChunks >= 2) ? op.fold_log2 : 0u;
+    const uint32_t fold_log2 = fold_tiles<kChunks, kMode>() ? op.fold_log2 : 0u;
 
     // rows of the tile: indptr
     if (op.indptr) {
