@@ -94,7 +94,7 @@ int launch_csr_build(const qr_op *op, const DeviceTables *t, uint64_t row_begin,
             uint32_t dbuf = 0;
             const uint32_t nh = heavy_fit(lf.hcnt, jw);
             const size_t sb = stage_bytes(g, jw, &dbuf);
-            const uint32_t fl = fold_tiles_log2(true, sb, heavy_bytes(nh, jw));
+            const uint32_t fl = fold_tiles_log2(true, sb, heavy_bytes(nh, jw), nt);
             const size_t shmem = sb + (heavy_bytes(nh, jw) << fl);
             TileOp top{t->gx + lf.s0, t->gterm + lf.s0, t->tz, t->tc, t->rl_ptr + lf.s0, t->rl,
                        d_indices, reinterpret_cast<double2 *>(d_data), nullptr, nullptr, nullptr,
@@ -132,7 +132,7 @@ int launch_csr_build(const qr_op *op, const DeviceTables *t, uint64_t row_begin,
     uint32_t dbuf = 0;
     const uint32_t nh = heavy_fit(t->nheavy, jw);
     const size_t sb = stage_bytes(G, jw, &dbuf);
-    const uint32_t fl = fold_tiles_log2(nch >= 2, sb, heavy_bytes(nh, jw));
+    const uint32_t fl = fold_tiles_log2(nch >= 2, sb, heavy_bytes(nh, jw), nt);
     const size_t shmem = sb + (heavy_bytes(nh, jw) << fl);
     TileOp top{t->gx, t->gterm, t->tz, t->tc, t->rl_ptr, t->rl, d_indices,
                reinterpret_cast<double2 *>(d_data), nullptr, nullptr, nullptr,

@@ -162,15 +162,23 @@ __host__ __device__ inline uint32_t heavy_fit(uint32_t nheavy, uint32_t jw_log2)
     while (nheavy && heavy_bytes(nheavy, jw_log2) > kHeavyMaxBytes) --nheavy;
     return nheavy;
 }
-// Tiles per block (log2: 2, 1 or 0) of a heavy-group instance that builds
-// several (fold_tiles), as many as keep the stage and their heavy values
-// within kFoldBudget bytes of LDS (those instances run one block per CU for
-// their registers).
+// Tiles per block (log2, at most kFoldMaxLog2) of a heavy-group instance that
+// builds several (fold_tiles), as many as keep the stage and their heavy
+// values within kFoldBudget bytes of LDS (those instances run one block per
+// CU for their registers) and leave at least kFoldMinBlocks blocks of the
+// launch's ntiles tiles, so that small operators still fill the chip.
+// up to 8 tiles: H12 (2^19 rows) 4.66 -> 4.42 ms against up to 4, H10 and
+// H11 unchanged; 16 do not fit the LDS budget there
+#ifndef QR_FOLD_MAX_LOG2
+#define QR_FOLD_MAX_LOG2 3
+#endif
 constexpr size_t kFoldBudget = 150 * 1024;
-__host__ __device__ inline uint32_t fold_tiles_log2(bool several, size_t stage, size_t heavy) {
+constexpr uint32_t kFoldMaxLog2 = QR_FOLD_MAX_LOG2;
+constexpr uint64_t kFoldMinBlocks = 512;
+__host__ __device__ inline uint32_t fold_tiles_log2(bool several, size_t stage, size_t heavy, uint64_t ntiles) {
     if (!several || heavy == 0) return 0;
-    uint32_t f = 2;
-    while (f && stage + (heavy << f) > kFoldBudget) --f;
+    uint32_t f = kFoldMaxLog2;
+    while (f && (stage + (heavy << f) > kFoldBudget || (ntiles >> f) < kFoldMinBlocks)) --f;
     return f;
 }
 
