@@ -245,6 +245,11 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
     // the second term of each group stays in registers when few chunks leave
     // room; with more than kStageMaxChunks not even the first one does
     constexpr bool kHold2 = kChunks <= 2;
+    // instances of operators with heavy groups (whose other groups run to 4-6
+    // terms) that run one block per CU also hold terms 2 and 3, loaded after
+    // the heavy fold so that they do not add to its registers (narrow masks:
+    // the 64-bit ones would spill)
+    constexpr bool kHold4 = kHold2 && kNarrow && fold_tiles<kChunks, kMode>();
     constexpr bool kLean = kChunks > kStageMaxChunks;
     const TileMem mem = tile_mem(op);
     const uint32_t G = op.G, jw_log2 = op.jw_log2, nchunks = op.nchunks;
@@ -374,6 +379,28 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
         }
         __syncthreads();
     }
+    mask_t z2[kChunks], z3[kChunks];
+    double2 c2[kChunks], c3[kChunks];
+#pragma unroll
+    for (int c = 0; c < kChunks; ++c) {
+        z2[c] = z3[c] = 0;
+        c2[c] = c3[c] = make_double2(0.0, 0.0);
+        const uint32_t j = (uint32_t(c) << jw_log2) + lane;
+        if (!kHold4 || uint32_t(c) >= nchunks || j >= G) continue;
+        // term 2 + u from the tail table (u < ntail) or the group's own range
+        auto hold = [&](uint32_t u, mask_t &z, double2 &cc) {
+            if (t1[c] <= t0[c] + 2 + u) return;
+            if (u < op.ntail) {
+                z = mask_t(mem.ttz[u * op.gstride + j]);
+                cc = ld2(mem.ttc + u * op.gstride + j);
+            } else {
+                z = mask_t(mem.tz[t0[c] + 2 + u]);
+                cc = ld2(mem.tc + t0[c] + 2 + u);
+            }
+        };
+        hold(0, z2[c], c2[c]);
+        hold(1, z3[c], c3[c]);
+    }
 
 #pragma unroll 1
     for (int m = 0; m < kBuildM; ++m) {
@@ -414,13 +441,28 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
                                                         parity_and<kNarrow>(r, mem.tz[tf + 1]));
                 v.x = v.x + e.x;
                 v.y = v.y + e.y;
+                uint32_t u0 = 0;
+                if (kHold4) {
+                    // terms 2 and 3 from registers
+                    if (t1[c] > tf + 2) {
+                        const double2 e2 = signed_coeff(c2[c], parity_and<kNarrow>(r, z2[c]));
+                        v.x = v.x + e2.x;
+                        v.y = v.y + e2.y;
+                    }
+                    if (t1[c] > tf + 3) {
+                        const double2 e3 = signed_coeff(c3[c], parity_and<kNarrow>(r, z3[c]));
+                        v.x = v.x + e3.x;
+                        v.y = v.y + e3.y;
+                    }
+                    u0 = 2;
+                }
                 // the remaining terms, in term order: from the transposed tail
                 // table while it has them -- the lanes, one group each, then
                 // read one contiguous run per position (u counts iterations,
                 // so the choice is the same for every active lane) -- else
                 // from the group's own term range.  Four loads in flight were
                 // slower than this serial loop, from either table.
-                for (uint32_t t = tf + 2, u = 0; t < t1[c]; ++t, ++u) {
+                for (uint32_t t = tf + 2 + u0, u = u0; t < t1[c]; ++t, ++u) {
                     double2 ct;
                     if (u < op.ntail) {
                         const uint32_t slot = u * op.gstride + j;
