@@ -250,6 +250,8 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
     // the heavy fold so that they do not add to its registers (narrow masks:
     // the 64-bit ones would spill)
     constexpr bool kHold4 = kHold2 && kNarrow && fold_tiles<kChunks, kMode>();
+    // the one-chunk sub-operators have room for terms 4 and 5 as well
+    constexpr bool kHold6 = kHold4 && kChunks == 1;
     constexpr bool kLean = kChunks > kStageMaxChunks;
     const TileMem mem = tile_mem(op);
     const uint32_t G = op.G, jw_log2 = op.jw_log2, nchunks = op.nchunks;
@@ -379,12 +381,12 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
         }
         __syncthreads();
     }
-    mask_t z2[kChunks], z3[kChunks];
-    double2 c2[kChunks], c3[kChunks];
+    mask_t z2[kChunks], z3[kChunks], z4[kChunks], z5[kChunks];
+    double2 c2[kChunks], c3[kChunks], c4[kChunks], c5[kChunks];
 #pragma unroll
     for (int c = 0; c < kChunks; ++c) {
-        z2[c] = z3[c] = 0;
-        c2[c] = c3[c] = make_double2(0.0, 0.0);
+        z2[c] = z3[c] = z4[c] = z5[c] = 0;
+        c2[c] = c3[c] = c4[c] = c5[c] = make_double2(0.0, 0.0);
         const uint32_t j = (uint32_t(c) << jw_log2) + lane;
         if (!kHold4 || uint32_t(c) >= nchunks || j >= G) continue;
         // term 2 + u from the tail table (u < ntail) or the group's own range
@@ -400,6 +402,10 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
         };
         hold(0, z2[c], c2[c]);
         hold(1, z3[c], c3[c]);
+        if (kHold6) {
+            hold(2, z4[c], c4[c]);
+            hold(3, z5[c], c5[c]);
+        }
     }
 
 #pragma unroll 1
@@ -455,6 +461,19 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
                         v.y = v.y + e3.y;
                     }
                     u0 = 2;
+                    if (kHold6) {
+                        if (t1[c] > tf + 4) {
+                            const double2 e4 = signed_coeff(c4[c], parity_and<kNarrow>(r, z4[c]));
+                            v.x = v.x + e4.x;
+                            v.y = v.y + e4.y;
+                        }
+                        if (t1[c] > tf + 5) {
+                            const double2 e5 = signed_coeff(c5[c], parity_and<kNarrow>(r, z5[c]));
+                            v.x = v.x + e5.x;
+                            v.y = v.y + e5.y;
+                        }
+                        u0 = 4;
+                    }
                 }
                 // the remaining terms, in term order: from the transposed tail
                 // table while it has them -- the lanes, one group each, then
