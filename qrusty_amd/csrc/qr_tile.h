@@ -243,8 +243,13 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
     constexpr bool kHeavy = kMode == kTileHeavy || kMode == kTileSubHeavy;
     constexpr bool kSub = kMode == kTileSub || kMode == kTileSubHeavy;
     // the second term of each group stays in registers when few chunks leave
-    // room; with more than kStageMaxChunks not even the first one does
-    constexpr bool kHold2 = kChunks <= 2;
+    // room; with more than kStageMaxChunks not even the first one does.  Not
+    // in the sub-operator mode without heavy groups (random operators of more
+    // than 2048 groups, and every operator of the batch's split form), whose
+    // groups rarely have a second term: there the held term cost the batch's
+    // two-chunk kernel 10 of its 14 spilled registers (256-operator batch
+    // 79.6-79.9 -> 78.0-78.3 ms)
+    constexpr bool kHold2 = kChunks <= 2 && kMode != kTileSub;
     // instances of operators with heavy groups (whose other groups run to 4-6
     // terms) that run one block per CU also hold terms 2 and 3, loaded after
     // the heavy fold so that they do not add to its registers (narrow masks:
