@@ -211,6 +211,50 @@ __device__ __forceinline__ double2 fold_group(const TileMem &mem, uint32_t j, ui
     return v;
 }
 
+// The same fold for NR rows of one group at once: each term is loaded once
+// for all of them, and the NR sums are independent chains (each in term
+// order, so every value equals fold_group's).
+template <bool kNarrow, uint32_t kF, int NR>
+__device__ __forceinline__ void fold_group_rows(const TileMem &mem, uint32_t j, const uint64_t (&r)[NR],
+                                                double2 (&v)[NR]) {
+    const uint32_t t0 = mem.gterm[j], t1 = mem.gterm[j + 1];
+    {
+        const double2 c = ld2(mem.tc + t0);
+        const uint64_t z = mem.tz[t0];
+#pragma unroll
+        for (int q = 0; q < NR; ++q) v[q] = signed_coeff(c, parity_and<kNarrow>(r[q], z));
+    }
+    uint32_t t = t0 + 1;
+    for (; t + kF <= t1; t += kF) {
+        double2 ct[kF];
+        uint64_t zt[kF];
+#pragma unroll
+        for (uint32_t u = 0; u < kF; ++u) {
+            ct[u] = ld2(mem.tc + t + u);
+            zt[u] = mem.tz[t + u];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kF; ++u) {
+#pragma unroll
+            for (int q = 0; q < NR; ++q) {
+                const double2 e = signed_coeff(ct[u], parity_and<kNarrow>(r[q], zt[u]));
+                v[q].x = v[q].x + e.x;
+                v[q].y = v[q].y + e.y;
+            }
+        }
+    }
+    for (; t < t1; ++t) {
+        const double2 c = ld2(mem.tc + t);
+        const uint64_t z = mem.tz[t];
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+            const double2 e = signed_coeff(c, parity_and<kNarrow>(r[q], z));
+            v[q].x = v[q].x + e.x;
+            v[q].y = v[q].y + e.y;
+        }
+    }
+}
+
 __host__ __device__ inline uint32_t stage_jw_log2(uint32_t G) {
     uint32_t jw = 0;
     while ((1u << jw) < G && jw < 10) ++jw;
@@ -379,10 +423,28 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
         // tile folds for the rows of all its tiles, so that the longest
         // chain, which sets the time of the pass, is paid once for them
         const uint32_t tr = uint32_t(kBuildM) << (rs_log2 + fold_log2);
-        for (uint32_t i = tid; i < op.nheavy * tr; i += kStageThreads) {
-            const uint64_t r = tile_base + (i & (tr - 1));
-            if (r >= row_begin && r < row_end)
-                hv[i] = fold_group<kNarrow, kChunks >= 2 ? 8u : 4u>(mem, mem.heavy[i / tr], r);
+        if constexpr (fold_tiles<kChunks, kMode>()) {
+            // (one block per CU) two rows of a group per thread, tr/2 apart:
+            // one load of each term for both, two independent add chains
+            constexpr int NR = 2;
+            const uint32_t half = tr / NR;
+            for (uint32_t i = tid; i < op.nheavy * half; i += kStageThreads) {
+                const uint32_t g = i / half, o = i % half;
+                uint64_t rr[NR];
+                double2 vv[NR];
+#pragma unroll
+                for (int q = 0; q < NR; ++q) rr[q] = tile_base + o + uint64_t(q) * half;
+                fold_group_rows<kNarrow, 4u, NR>(mem, mem.heavy[g], rr, vv);
+#pragma unroll
+                for (int q = 0; q < NR; ++q)
+                    if (rr[q] >= row_begin && rr[q] < row_end) hv[g * tr + o + uint32_t(q) * half] = vv[q];
+            }
+        } else {
+            for (uint32_t i = tid; i < op.nheavy * tr; i += kStageThreads) {
+                const uint64_t r = tile_base + (i & (tr - 1));
+                if (r >= row_begin && r < row_end)
+                    hv[i] = fold_group<kNarrow, kChunks >= 2 ? 8u : 4u>(mem, mem.heavy[i / tr], r);
+            }
         }
         __syncthreads();
     }
