@@ -93,6 +93,8 @@ int launch_csr_build(const qr_op *op, const DeviceTables *t, uint64_t row_begin,
             const uint32_t g = lf.s1 - lf.s0, nch = (g + 1023) >> 10;
             uint32_t dbuf = 0;
             const uint32_t nh = heavy_fit(lf.hcnt, jw);
+            // (sub-operators keep one stage: the second would cost them
+            // tiles per heavy fold, H10 14.12 -> 14.38 ms)
             const size_t sb = stage_bytes(g, jw, &dbuf);
             const uint32_t fl = fold_tiles_log2(true, sb, heavy_bytes(nh, jw), nt);
             const size_t shmem = sb + (heavy_bytes(nh, jw) << fl);
@@ -131,7 +133,7 @@ int launch_csr_build(const qr_op *op, const DeviceTables *t, uint64_t row_begin,
     if (nt > 0x7FFFFFFFull) return fail(QR_ERR_SIZE, "CSR build grid too large");
     uint32_t dbuf = 0;
     const uint32_t nh = heavy_fit(t->nheavy, jw);
-    const size_t sb = stage_bytes(G, jw, &dbuf);
+    const size_t sb = nh ? heavy_stage(nch >= 2, G, jw, heavy_bytes(nh, jw), &dbuf) : stage_bytes(G, jw, &dbuf);
     const uint32_t fl = fold_tiles_log2(nch >= 2, sb, heavy_bytes(nh, jw), nt);
     const size_t shmem = sb + (heavy_bytes(nh, jw) << fl);
     TileOp top{t->gx, t->gterm, t->tz, t->tc, t->rl_ptr, t->rl, d_indices,

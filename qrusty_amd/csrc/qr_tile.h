@@ -143,12 +143,15 @@ __device__ __forceinline__ double2 ld2(gptr<const dvec2> p) {
 #ifndef QR_STAGE_DBUF_MAX
 #define QR_STAGE_DBUF_MAX 0
 #endif
-__host__ __device__ inline size_t stage_bytes(uint32_t G, uint32_t jw_log2, uint32_t *dbuf_out) {
+__host__ __device__ inline size_t stage_bytes_for(uint32_t G, uint32_t jw_log2, uint32_t dbuf) {
     const size_t batch_pad = ((size_t(G) << (10 - jw_log2)) + 1) & ~size_t(1);
     const size_t one = batch_pad * (sizeof(uint64_t) + sizeof(double2));
-    const uint32_t dbuf = 2 * one <= QR_STAGE_DBUF_MAX ? 1u : 0u;
-    if (dbuf_out) *dbuf_out = dbuf;
     return (dbuf ? 2 * one : one) + 2 * 16 * sizeof(double2);
+}
+__host__ __device__ inline size_t stage_bytes(uint32_t G, uint32_t jw_log2, uint32_t *dbuf_out) {
+    const uint32_t dbuf = stage_bytes_for(G, jw_log2, 1) - 2 * 16 * sizeof(double2) <= QR_STAGE_DBUF_MAX ? 1u : 0u;
+    if (dbuf_out) *dbuf_out = dbuf;
+    return stage_bytes_for(G, jw_log2, dbuf);
 }
 
 // LDS bytes of the heavy-group values: one double2 per row of the tile and
@@ -173,6 +176,20 @@ __host__ __device__ inline uint32_t heavy_fit(uint32_t nheavy, uint32_t jw_log2)
 #define QR_FOLD_MAX_LOG2 3
 #endif
 constexpr size_t kFoldBudget = 150 * 1024;
+// The two-chunk whole-operator instances (one block per CU, so no second
+// block overlaps a batch's copy-out with the next batch's build)
+// double-buffer the stage when it and one tile's heavy values fit the
+// budget; the tiles per block are then chosen with the double stage.
+// Measured on H12 (2^19 rows): 3.93 -> 3.72-3.76 ms.
+#ifndef QR_HEAVY_DBUF
+#define QR_HEAVY_DBUF 1
+#endif
+__host__ __device__ inline size_t heavy_stage(bool several, uint32_t G, uint32_t jw_log2, size_t heavy,
+                                              uint32_t *dbuf) {
+    *dbuf = 0;
+    if (QR_HEAVY_DBUF && several && heavy && stage_bytes_for(G, jw_log2, 1) + heavy <= kFoldBudget) *dbuf = 1;
+    return stage_bytes_for(G, jw_log2, *dbuf);
+}
 constexpr uint32_t kFoldMaxLog2 = QR_FOLD_MAX_LOG2;
 constexpr uint64_t kFoldMinBlocks = 512;
 __host__ __device__ inline uint32_t fold_tiles_log2(bool several, size_t stage, size_t heavy, uint64_t ntiles) {
@@ -324,7 +341,7 @@ __device__ __forceinline__ void build_tile_staged(const TileOp &op, uint64_t til
     double2 *red = reinterpret_cast<double2 *>(smem_raw + (op.dbuf ? 2u : 1u) * stage_b);
     // heavy-group values of the rows of the block's tiles:
     // hv[h << (tile_log2 + fold_log2) | sub << tile_log2 | (r - tile_base)]
-    double2 *hv = reinterpret_cast<double2 *>(smem_raw + stage_bytes(G, jw_log2, nullptr));
+    double2 *hv = reinterpret_cast<double2 *>(smem_raw + stage_bytes_for(G, jw_log2, op.dbuf));
     const uint32_t fold_log2 = fold_tiles<kChunks, kMode>() ? op.fold_log2 : 0u;
 
     // rows of the tile: indptr
