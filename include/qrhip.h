@@ -321,10 +321,13 @@ int qr_pauli_apply(const qr_op *op, int32_t precision, uint32_t nvec, const void
  *  - the <x|y> block partials live in a file-scope device array of 4 MiB per
  *    device (2^19 doubles): 2 * nvec * (2^n >> L) doubles must fit, with
  *    L = 11 (QR_PREC_MIXED) or 10 (QR_PREC_FP64) -- e.g. 8 states of 24
- *    qubits, or one of 28 -- else QR_ERR_SIZE;
+ *    qubits, or one of 28 -- else QR_ERR_SIZE (complex64 launches of fewer
+ *    than 4 tiles per CU take tiles of 2^10 rows, whose partials always fit);
  *  - the operator's device tables for this precision must already be on the
  *    device: one qr_pauli_apply call of the same operator and precision on
- *    that device uploads them (QR_ERR_ARG otherwise).
+ *    that device uploads them (QR_ERR_ARG otherwise).  Complex64 has tables
+ *    per tile size; when only the other size's are on the device, the call
+ *    uses those (same results as the qr_pauli_apply that uploaded them).
  * Every call on one device shares that array, which keeps the last call's
  * contents: calls that overlap on one device (other streams or threads)
  * race, so the caller serialises them. */

@@ -290,6 +290,11 @@ int device_tables(const qr_op *op, DeviceTables **out, bool noalloc) {
     return QR_OK;
 }
 
+bool tables_ready(const qr_op *op, const DeviceTables *t, int slot, int L) {
+    std::lock_guard<std::mutex> lock(op->mu);
+    return t->pat[slot].L == L;
+}
+
 int device_patterns(const qr_op *op, DeviceTables *t, int slot, int L, int tb, uint32_t pcap,
                     uint32_t ucap, bool staged, bool pairs, bool noalloc) {
     std::lock_guard<std::mutex> lock(op->mu);

@@ -87,12 +87,22 @@ constexpr int kApplyThreadBits = 8;
 constexpr int kCompBlock = 4;
 constexpr size_t kApplyMaxLDS = 40 * 1024;  // LDS of one block
 
-// Tile of 2^L rows (x tile in LDS: 16 KB in both precisions), R rows per
-// thread, and the LDS left for one chunk of pattern coefficients (20 B each)
-// and the row-bit table (R x 16 B per row, plus a zero row).
-template <bool kMixed>
+// Tile of 2^L rows (x tile in LDS: 16 KB in both precisions at the default
+// L; 8 KB for the small complex64 tiles), R rows per thread, and the LDS left
+// for one chunk of pattern coefficients (20 B each) and the row-bit table
+// (R x 16 B per row, plus a zero row).
+//
+// Small complex64 tiles (2^10 rows, 4 per thread) serve launches that would
+// leave CUs short of blocks with 2^11-row tiles: the block keeps the latency
+// of its LDS and gather round trips hidden only with others beside it.
+// Measured, one state (20 qubits, 14251 terms in 2536 groups; 512 blocks of
+// 2^11 rows, two per CU): 2.29 -> 1.70 ms.  Where the default tiles fill the
+// chip they are 20-27% slower (22- and 24-qubit operators of 811-3792
+// groups, the 20-qubit one with 4 states, 24-qubit Heisenberg with 8).
+constexpr int kMixedSmallL = 10;
+template <bool kMixed, int kL = kMixed ? 11 : 10>
 struct ApplyShape {
-    static constexpr int L = kMixed ? 11 : 10;
+    static constexpr int L = kL;
     static constexpr int R = (1 << L) / kApplyThreads;
     static constexpr size_t kTile = (size_t(1) << L) * (kMixed ? 8 : 16);
     // LDS after the x tile: 5/8 for patterns, 3/8 for the row-bit table
@@ -453,7 +463,7 @@ __device__ __forceinline__ void chunk_tables(const ApplyChunk &ch, uint32_t tid,
     __syncthreads();
 }
 
-template <bool kMixed, bool kNarrow, bool kFull>
+template <bool kMixed, bool kNarrow, bool kFull, int kL>
 __global__ __launch_bounds__(kApplyThreads) void k_apply_tiled(
     uint64_t N, uint32_t ntiles, uint32_t tiles_per_xcd, uint32_t ngrp, uint32_t nchunk,
     uint32_t max_p, uint32_t zero_row, const ApplyGroup *__restrict__ grp, const ApplyChunk *__restrict__ chunks,
@@ -461,7 +471,7 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_tiled(
     const uint32_t *__restrict__ ptptr, const uint64_t *__restrict__ ptz,
     const double2 *__restrict__ ptc, const void *__restrict__ xin, void *__restrict__ yout,
     double *__restrict__ partials) {
-    using Shape = ApplyShape<kMixed>;
+    using Shape = ApplyShape<kMixed, kL>;
     constexpr int L = Shape::L, R = Shape::R;
     // complex64 full tiles: paired rows (see gather); the patterns are built
     // in the paired-row bit order to match
@@ -625,7 +635,6 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_tiled(
 // the tile itself), so a partner tile is read once per chunk however many
 // groups use it.  The load of run r + 1 (LDS-DMA: global memory straight into
 // LDS, no registers) is in flight while run r is computed.
-constexpr uint32_t kStageTileBytes = 16384;  // one tile: 2^11 complex64 or 2^10 complex128
 
 // One LDS-DMA load: each lane reads the 16 bytes at its `src`, and the wave
 // writes them to LDS at lds + lane * 16 (lds: LDS byte address, wave-uniform,
@@ -653,7 +662,7 @@ __device__ __forceinline__ void lds_dma16(const void *src, uint32_t lds) {
 
 typedef __attribute__((address_space(3))) unsigned char lds_byte;
 
-template <bool kMixed, bool kNarrow>
+template <bool kMixed, bool kNarrow, int kL>
 __global__ __launch_bounds__(kApplyThreads) void k_apply_staged(
     uint64_t N, uint32_t ntiles, uint32_t tiles_per_xcd, uint32_t nchunk, uint32_t nrun,
     uint32_t max_p, uint32_t zero_row, const ApplyGroup *__restrict__ grp,
@@ -662,10 +671,11 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_staged(
     const uint32_t *__restrict__ ptptr, const uint64_t *__restrict__ ptz,
     const double2 *__restrict__ ptc, const void *__restrict__ xin, void *__restrict__ yout,
     double *__restrict__ partials) {
-    using Shape = ApplyShape<kMixed>;
+    using Shape = ApplyShape<kMixed, kL>;
     constexpr int L = Shape::L, R = Shape::R;
     using cT = typename std::conditional<kMixed, float2, double2>::type;
     constexpr uint64_t tile_rows = uint64_t(1) << L;
+    constexpr uint32_t kStageTileBytes = uint32_t(Shape::kTile);  // one stage buffer: one tile
     constexpr uint32_t kLoads = kStageTileBytes / (16u * kApplyThreads);  // 16-byte loads per thread
     static_assert(tile_rows * sizeof(cT) == kStageTileBytes, "a tile fills one stage buffer");
     static_assert(kStageTileBytes % (16u * kApplyThreads) == 0, "tile must split into 16-byte loads");
@@ -830,21 +840,21 @@ __global__ __launch_bounds__(256) void k_reduce_partials(uint32_t ntiles,
     if (threadIdx.x == 0) { out[2 * v] = ra[0]; out[2 * v + 1] = rb[0]; }
 }
 
-template <bool kMixed, bool kNarrow>
-static int launch_tiled(const qr_op *op, DeviceTables *t, uint32_t nvec, const void *d_x,
+template <bool kMixed, bool kNarrow, int kL>
+static int launch_tiled(const qr_op *op, DeviceTables *t, int slot, uint32_t nvec, const void *d_x,
                         void *d_y, double *partials, uint32_t ntiles, hipStream_t s) {
-    using Shape = ApplyShape<kMixed>;
+    using Shape = ApplyShape<kMixed, kL>;
     using cT = typename std::conditional<kMixed, float2, double2>::type;
     const uint64_t N = uint64_t(1) << op->nq;
     const uint64_t tile_rows = N < (uint64_t(1) << Shape::L) ? N : (uint64_t(1) << Shape::L);
-    const DeviceTables::Patterns &pt = t->pat[kMixed ? 1 : 0];
+    const DeviceTables::Patterns &pt = t->pat[slot];
     // the row-bit table holds max_u rows plus one zero row
     const size_t shmem = apply_lds_offsets<cT>(tile_rows, pt.max_p, Shape::R, pt.max_u + 1,
                                                nullptr, nullptr, nullptr);
     if (shmem > kApplyMaxLDS) return fail(QR_ERR_SIZE, "apply: LDS stage too large");
     const uint32_t tpx = (ntiles % 8 == 0) ? ntiles / 8 : 0;
     const bool full = tile_rows == (uint64_t(1) << Shape::L);
-    auto kern = full ? k_apply_tiled<kMixed, kNarrow, true> : k_apply_tiled<kMixed, kNarrow, false>;
+    auto kern = full ? k_apply_tiled<kMixed, kNarrow, true, kL> : k_apply_tiled<kMixed, kNarrow, false, kL>;
     QR_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem)));
     const uint64_t nblocks = uint64_t(ntiles) * nvec;
@@ -856,20 +866,20 @@ static int launch_tiled(const qr_op *op, DeviceTables *t, uint32_t nvec, const v
     return QR_OK;
 }
 
-template <bool kMixed, bool kNarrow>
-static int launch_staged(const qr_op *op, DeviceTables *t, uint32_t nvec, const void *d_x,
+template <bool kMixed, bool kNarrow, int kL>
+static int launch_staged(const qr_op *op, DeviceTables *t, int slot, uint32_t nvec, const void *d_x,
                          void *d_y, double *partials, uint32_t ntiles, hipStream_t s) {
-    using Shape = ApplyShape<kMixed>;
+    using Shape = ApplyShape<kMixed, kL>;
     using cT = typename std::conditional<kMixed, float2, double2>::type;
     const uint64_t N = uint64_t(1) << op->nq;
-    const DeviceTables::Patterns &pt = t->pat[2 + (kMixed ? 1 : 0)];
+    const DeviceTables::Patterns &pt = t->pat[slot];
     // dynamic LDS holds the tables; the two tile buffers are static
     const size_t shmem = apply_lds_offsets<cT>(0, pt.max_p, Shape::R, pt.max_u + 1, nullptr,
                                                nullptr, nullptr);
-    if (shmem + 2 * kStageTileBytes + 1024 > kApplyMaxLDS)
+    if (shmem + 2 * Shape::kTile + 1024 > kApplyMaxLDS)
         return fail(QR_ERR_SIZE, "apply: LDS stage too large");
     const uint32_t tpx = (ntiles % 8 == 0) ? ntiles / 8 : 0;
-    auto kern = k_apply_staged<kMixed, kNarrow>;
+    auto kern = k_apply_staged<kMixed, kNarrow, kL>;
     QR_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem)));
     const uint64_t nblocks = uint64_t(ntiles) * nvec;
@@ -973,6 +983,47 @@ static int scratch_partials(double **out) {
     return QR_OK;
 }
 
+// Compute units of the current device (queried once per device).
+static int apply_cus(int *out) {
+    static std::mutex mu;
+    static int cache[kMaxDevices];
+    int dev = 0;
+    QR_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(QR_ERR_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!cache[dev]) {
+        int cus = 0;
+        QR_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        cache[dev] = cus > 0 ? cus : 1;
+    }
+    *out = cache[dev];
+    return QR_OK;
+}
+
+// One way of applying an operator: tile size, form, and the pattern-table
+// slot that holds its tables (2 * kind + staged; kind 0: FP64, 1: complex64,
+// 2: small complex64 tiles).
+struct ApplyPlan {
+    int L = 0, slot = 0;
+    bool staged = false, pairs = false, small = false;
+    uint32_t pcap = 0, ucap = 0;
+};
+
+template <bool kMixed, int kL>
+static ApplyPlan make_plan(const qr_op *op, int kind) {
+    using Shape = ApplyShape<kMixed, kL>;
+    ApplyPlan p;
+    p.L = int(op->nq) < kL ? int(op->nq) : kL;
+    p.staged = p.L == kL && prefer_staged(op, p.L, kMixed ? 8 : 16);
+    p.pcap = p.staged ? Shape::kPatChunkS : Shape::kPatChunk;
+    p.ucap = p.staged ? Shape::kTableRowsS : Shape::kTableRows;
+    // the gather form on full complex64 tiles runs the paired-row layout
+    p.pairs = kMixed && !p.staged && p.L == kL;
+    p.small = kind == 2;
+    p.slot = 2 * kind + (p.staged ? 1 : 0);
+    return p;
+}
+
 int launch_pauli_apply(const qr_op *op, DeviceTables *t, int32_t precision, uint32_t nvec,
                        const void *d_x, void *d_y, double *d_expect, hipStream_t s,
                        bool noalloc) {
@@ -981,21 +1032,40 @@ int launch_pauli_apply(const qr_op *op, DeviceTables *t, int32_t precision, uint
     const char *wide_env = getenv("QR_APPLY_WIDE");
     const bool force_wide = wide_env && atoi(wide_env) == 1;
     const bool mixed = precision == QR_PREC_MIXED, narrow = op->nq <= 32 && !force_wide;
-    const int Lmax = mixed ? ApplyShape<true>::L : ApplyShape<false>::L;
-    const int L = int(op->nq) < Lmax ? int(op->nq) : Lmax;
-    const bool staged = L == Lmax && prefer_staged(op, L, mixed ? 8 : 16);
-    const uint32_t pcap = staged ? (mixed ? ApplyShape<true>::kPatChunkS : ApplyShape<false>::kPatChunkS)
-                                 : (mixed ? ApplyShape<true>::kPatChunk : ApplyShape<false>::kPatChunk);
-    const uint32_t ucap = staged ? (mixed ? ApplyShape<true>::kTableRowsS : ApplyShape<false>::kTableRowsS)
-                                 : (mixed ? ApplyShape<true>::kTableRows : ApplyShape<false>::kTableRows);
-    // the gather form on full complex64 tiles runs the paired-row layout
-    const bool pairs = mixed && !staged && L == Lmax;
-    int rc = device_patterns(op, t, (mixed ? 1 : 0) + (staged ? 2 : 0), L, kApplyThreadBits, pcap,
-                             ucap, staged, pairs, noalloc);
-    if (rc) return rc;
     const uint64_t N = uint64_t(1) << op->nq;
-    if ((N >> L) * uint64_t(nvec) > 0x7FFFFFFFull || nvec == 0)
-        return fail(QR_ERR_SIZE, "apply: state too large for one launch (or nvec == 0)");
+    if (nvec == 0) return fail(QR_ERR_SIZE, "apply: nvec == 0");
+    int rc;
+    ApplyPlan plan;
+    if (!mixed) {
+        plan = make_plan<false, ApplyShape<false>::L>(op, 0);
+    } else {
+        // small complex64 tiles when the default ones give fewer than four
+        // blocks per CU (the blocks one CU holds at once); QR_APPLY_SMALL
+        // (0 never, 1 always where the state has more rows than a small
+        // tile) overrides the choice, for tests and measurement
+        int cus = 0;
+        if ((rc = apply_cus(&cus))) return rc;
+        const bool can_small = int(op->nq) > kMixedSmallL;
+        bool small = can_small && (N >> ApplyShape<true>::L) * nvec < uint64_t(4) * uint64_t(cus);
+        const char *small_env = getenv("QR_APPLY_SMALL");
+        if (small_env) small = can_small && atoi(small_env) == 1;
+        plan = small ? make_plan<true, kMixedSmallL>(op, 2) : make_plan<true, ApplyShape<true>::L>(op, 1);
+        // without allocation only uploaded tables can serve: those of the
+        // other tile size when this one's are not on the device
+        if (noalloc && can_small) {
+            const ApplyPlan other = small ? make_plan<true, ApplyShape<true>::L>(op, 1)
+                                          : make_plan<true, kMixedSmallL>(op, 2);
+            if (!tables_ready(op, t, plan.slot, plan.L) && tables_ready(op, t, other.slot, other.L))
+                plan = other;
+        }
+    }
+    rc = device_patterns(op, t, plan.slot, plan.L, kApplyThreadBits, plan.pcap, plan.ucap,
+                         plan.staged, plan.pairs, noalloc);
+    if (rc) return rc;
+    const int L = plan.L;
+    const bool staged = plan.staged;
+    if ((N >> L) * uint64_t(nvec) > 0x7FFFFFFFull)
+        return fail(QR_ERR_SIZE, "apply: state too large for one launch");
     const uint32_t ntiles = uint32_t(N >> L);
     // <x|y> block partials: a buffer of this launch's own, taken from the
     // device's pool and given back after the reduction, both in stream order,
@@ -1016,15 +1086,28 @@ int launch_pauli_apply(const qr_op *op, DeviceTables *t, int32_t precision, uint
         QR_HIP_CHECK(hipMallocFromPoolAsync(reinterpret_cast<void **>(&partials),
                                             size_t(ntiles) * nvec * 2 * sizeof(double), pool, s));
     }
+    constexpr int kLm = ApplyShape<true>::L, kLd = ApplyShape<false>::L, kLs = kMixedSmallL;
+    const int sl = plan.slot;
     if (staged) {
-        if (mixed) rc = narrow ? launch_staged<true, true>(op, t, nvec, d_x, d_y, partials, ntiles, s)
-                               : launch_staged<true, false>(op, t, nvec, d_x, d_y, partials, ntiles, s);
-        else rc = narrow ? launch_staged<false, true>(op, t, nvec, d_x, d_y, partials, ntiles, s)
-                         : launch_staged<false, false>(op, t, nvec, d_x, d_y, partials, ntiles, s);
-    } else if (mixed) rc = narrow ? launch_tiled<true, true>(op, t, nvec, d_x, d_y, partials, ntiles, s)
-                                  : launch_tiled<true, false>(op, t, nvec, d_x, d_y, partials, ntiles, s);
-    else rc = narrow ? launch_tiled<false, true>(op, t, nvec, d_x, d_y, partials, ntiles, s)
-                     : launch_tiled<false, false>(op, t, nvec, d_x, d_y, partials, ntiles, s);
+        if (mixed && plan.small)
+            rc = narrow ? launch_staged<true, true, kLs>(op, t, sl, nvec, d_x, d_y, partials, ntiles, s)
+                        : launch_staged<true, false, kLs>(op, t, sl, nvec, d_x, d_y, partials, ntiles, s);
+        else if (mixed)
+            rc = narrow ? launch_staged<true, true, kLm>(op, t, sl, nvec, d_x, d_y, partials, ntiles, s)
+                        : launch_staged<true, false, kLm>(op, t, sl, nvec, d_x, d_y, partials, ntiles, s);
+        else
+            rc = narrow ? launch_staged<false, true, kLd>(op, t, sl, nvec, d_x, d_y, partials, ntiles, s)
+                        : launch_staged<false, false, kLd>(op, t, sl, nvec, d_x, d_y, partials, ntiles, s);
+    } else if (mixed && plan.small) {
+        rc = narrow ? launch_tiled<true, true, kLs>(op, t, sl, nvec, d_x, d_y, partials, ntiles, s)
+                    : launch_tiled<true, false, kLs>(op, t, sl, nvec, d_x, d_y, partials, ntiles, s);
+    } else if (mixed) {
+        rc = narrow ? launch_tiled<true, true, kLm>(op, t, sl, nvec, d_x, d_y, partials, ntiles, s)
+                    : launch_tiled<true, false, kLm>(op, t, sl, nvec, d_x, d_y, partials, ntiles, s);
+    } else {
+        rc = narrow ? launch_tiled<false, true, kLd>(op, t, sl, nvec, d_x, d_y, partials, ntiles, s)
+                    : launch_tiled<false, false, kLd>(op, t, sl, nvec, d_x, d_y, partials, ntiles, s);
+    }
     if (!rc && d_expect) {
         hipLaunchKernelGGL(k_reduce_partials, dim3(nvec), dim3(256), 0, s, ntiles, partials,
                            d_expect);

@@ -117,8 +117,8 @@ struct DeviceTables {
     double2 *ttc = nullptr;
     uint32_t ntail = 0;
     uint32_t *leaf_heavy = nullptr;  // qr_op::leaf_heavy
-    // matrix-free apply tables, one set per precision (tile size) and form
-    // (slot = precision + 2 * staged)
+    // matrix-free apply tables, one set per tile kind and form (slot =
+    // 2 * kind + staged; kind 0: FP64, 1: complex64, 2: small complex64 tiles)
     struct Patterns {
         int L = -1;
         uint32_t P = 0, ngrp = 0, nchunk = 0, nrun = 0, max_p = 0, max_u = 0;
@@ -131,7 +131,7 @@ struct DeviceTables {
         uint32_t *ptptr = nullptr;  // [P+1] term range of each pattern
         uint64_t *ptz = nullptr;    // [T]   z masks in pattern order
         double2 *ptc = nullptr;     // [T]   c' in pattern order
-    } pat[4];
+    } pat[6];
 };
 
 // Host build of the apply tables for tile size L, tb thread-index bits, at
@@ -158,6 +158,8 @@ inline uint64_t pair_row_bits(uint64_t zl, int tb) {
 }
 void build_patterns(const qr_op *op, int L, int tb, uint32_t pcap, uint32_t ucap, bool staged,
                     bool pairs, HostPatterns &out);
+// Whether slot `slot` holds the apply tables for tile size L.
+bool tables_ready(const qr_op *op, const DeviceTables *t, int slot, int L);
 // With `noalloc` the tables must already be on the device (QR_ERR_ARG if not):
 // nothing is allocated or uploaded.
 int device_patterns(const qr_op *op, DeviceTables *t, int slot, int L, int tb, uint32_t pcap,

@@ -210,7 +210,9 @@ def test_apply_forms_vs_oracle(monkeypatch, kind, n, T, seed):
     x = _states(p.N, nvec, seed=seed)
     rows = np.random.default_rng(seed).choice(p.N, size=2048, replace=False).astype(np.uint64)
     ref = orc.apply_rows(p, x, rows)
-    eref = orc.vdot(x, orc.apply_full(p, x))
+    yfull = orc.apply_full(p, x)
+    eref = orc.vdot(x, yfull)
+    escale = np.linalg.norm(x, axis=1) * np.linalg.norm(yfull, axis=1)  # |<x|Hx>| <= this
     for form in ("gather", "staged"):
         monkeypatch.setenv("QR_APPLY_FORM", "1" if form == "gather" else "2")
         print(f"[gpu] apply form={form} {kind} n={n} T={len(labels)} seed={seed}", flush=True)
@@ -221,10 +223,15 @@ def test_apply_forms_vs_oracle(monkeypatch, kind, n, T, seed):
         assert np.all(np.abs(got - ref) <= 1e-12 * _abs_bound(p, x, rows) + 1e-300)
         assert np.all(np.abs(e - eref) <= 1e-12 * np.abs(eref) + 1e-14)
         assert np.array_equal(op.expectation(x, "fp64"), e)  # y never written
-        ym, em = op.apply_and_expectation(x, "mixed")
-        ry = np.linalg.norm(ym[:, rows.astype(np.int64)] - ref) / np.linalg.norm(ref)
-        print(f"[gpu]   mixed rel err y {ry:.2e}", flush=True)
-        assert ry <= 1e-6
+        # complex64: default tiles (2^11 rows) and small ones (2^10)
+        for small in ("0", "1"):
+            monkeypatch.setenv("QR_APPLY_SMALL", small)
+            ym, em = op.apply_and_expectation(x, "mixed")
+            ry = np.linalg.norm(ym[:, rows.astype(np.int64)] - ref) / np.linalg.norm(ref)
+            print(f"[gpu]   mixed (small tiles {small}) rel err y {ry:.2e}", flush=True)
+            assert ry <= 1e-6
+            assert np.all(np.abs(em - eref) <= 1e-6 * escale)
+        monkeypatch.delenv("QR_APPLY_SMALL")
 
 
 @pytest.mark.parametrize("form", ["1", "2"])
@@ -246,8 +253,10 @@ def test_wide_mask_instances(monkeypatch, form):
     assert np.all(np.abs(y - ref) <= 1e-12 * _abs_bound(p, x, rows) + 1e-300)
     eref = orc.vdot(x, orc.apply_full(p, x))
     assert np.all(np.abs(e - eref) <= 1e-12 * np.abs(eref) + 1e-14)
-    ym = op.apply(x, "mixed")
-    assert np.linalg.norm(ym - ref) <= 1e-6 * np.linalg.norm(ref)
+    for small in ("0", "1"):
+        monkeypatch.setenv("QR_APPLY_SMALL", small)
+        ym = op.apply(x, "mixed")
+        assert np.linalg.norm(ym - ref) <= 1e-6 * np.linalg.norm(ref)
 
 
 def _label_from_masks(n, xm, zm):
@@ -260,13 +269,15 @@ def _label_from_masks(n, xm, zm):
     return "".join(chars)
 
 
+@pytest.mark.parametrize("small", ["0", "1"])
 @pytest.mark.parametrize("n,seed", [(11, 91), (12, 92), (13, 93)])
-def test_mixed_gather_row_pairs(monkeypatch, n, seed):
+def test_mixed_gather_row_pairs(monkeypatch, n, seed, small):
     """The complex64 gather form on full tiles takes rows in adjacent pairs,
     with the patterns' low z bits permuted to match.  Masks here set row bit 0
     (pair swap), the thread bits 1..8 and the row bits 9..10 on both sides of
     that split, with complex coefficients; every row is compared with the
-    oracle, normwise and componentwise."""
+    oracle, normwise and componentwise.  On tiles of 2^11 rows (QR_APPLY_SMALL
+    = 0) and of 2^10 (= 1: bit 10 then reaches the partner tile)."""
     rng = np.random.default_rng(seed)
     picks = [0, 1, 2, 3, 0x100, 0x101, 0x200, 0x201, 0x300, 0x401, 0x600, 0x603, 0x7FF]
     hi = [1 << b for b in range(11, n)]
@@ -286,7 +297,8 @@ def test_mixed_gather_row_pairs(monkeypatch, n, seed):
     ref = orc.apply_rows(p, x, rows)
     eref = orc.vdot(x, ref)
     monkeypatch.setenv("QR_APPLY_FORM", "1")
-    print(f"[gpu] apply mixed gather row pairs n={n} T={len(labels)} seed={seed}", flush=True)
+    monkeypatch.setenv("QR_APPLY_SMALL", small)
+    print(f"[gpu] apply mixed gather row pairs n={n} T={len(labels)} seed={seed} small={small}", flush=True)
     op = _op(labels, coeffs)
     ym, em = op.apply_and_expectation(x, "mixed")
     ry = np.linalg.norm(ym - ref) / np.linalg.norm(ref)

@@ -63,6 +63,28 @@ def test_noalloc_equals_apply_two_calls(hamiltonians, prec_name):
         assert np.all(np.isfinite(e))
 
 
+def test_noalloc_other_tile_size(monkeypatch, hamiltonians):
+    """Complex64 applies that would leave CUs short of blocks take tiles of
+    2^10 rows instead of 2^11, each size with tables of its own.  When only
+    the other size's tables are on the device, the allocation-free call uses
+    those, and equals the allocating call that uploaded them bit for bit."""
+    from qrusty_amd import _lib as L
+    from qrusty_amd import workloads
+    labels, coeffs = hamiltonians["py_H8"]  # 16 qubits: 32 tiles of 2^11 rows
+    op, h = _handle(labels, coeffs)
+    x = torch.from_numpy(workloads.random_states(1 << h.num_qubits, 2, 54)).to("cuda", torch.complex64)
+    print("[gpu] noalloc mixed, tables of the other tile size", flush=True)
+    monkeypatch.setenv("QR_APPLY_SMALL", "0")
+    y_ref, e_ref = _run(L.lib.qr_pauli_apply, h, L.QR_PREC_MIXED, x, True)  # 2^11-row tables
+    monkeypatch.delenv("QR_APPLY_SMALL")
+    y, e = _run(L.lib.qr_pauli_apply_noalloc, h, L.QR_PREC_MIXED, x, True)  # would take 2^10
+    assert np.array_equal(y, y_ref) and np.array_equal(e, e_ref)
+    y2_ref, e2_ref = _run(L.lib.qr_pauli_apply, h, L.QR_PREC_MIXED, x, True)  # uploads 2^10
+    y2, e2 = _run(L.lib.qr_pauli_apply_noalloc, h, L.QR_PREC_MIXED, x, True)
+    assert np.array_equal(y2, y2_ref) and np.array_equal(e2, e2_ref)
+    assert np.linalg.norm(y2 - y) <= 1e-6 * np.linalg.norm(y)
+
+
 def test_noalloc_refusals(hamiltonians):
     from qrusty_amd import _lib as L
     labels, coeffs = hamiltonians["py_H8"]  # 16 qubits: 64 FP64 tiles
