@@ -71,15 +71,27 @@ __device__ __forceinline__ void merge_row(uint64_t r, const SpAddIn &in, Emit em
     }
 }
 
-#ifndef QR_SPADD_THREADS
-#define QR_SPADD_THREADS 256
-#endif
 #ifndef QR_SPADD_CAP
 #define QR_SPADD_CAP 512
 #endif
-constexpr int kSpThreads = QR_SPADD_THREADS;
+// Threads per block: measured on the 24-qubit H - e*I (count + fill, ms) with
+// 512-entry tiles, 256 threads 3.2 + 5.4, 128 threads 2.7 + 6.3.  The two
+// passes need not share a tiling (for a CSR whose indptr ascends within nnz,
+// each route of the merge gives a row the same entries), so each takes its
+// faster one.  Count pass, threads / tile entries: 128 / 512 2.52-2.64 ms,
+// 64 / 256 2.60, 64 / 512 2.85, 256 / 1024 2.98, 128 / 256 3.10,
+// 128 / 1024 3.45.
+#ifndef QR_SPADD_COUNT_THREADS
+#define QR_SPADD_COUNT_THREADS 128
+#endif
+#ifndef QR_SPADD_COUNT_CAP
+#define QR_SPADD_COUNT_CAP 512
+#endif
+constexpr int kSpCountThreads = QR_SPADD_COUNT_THREADS, kSpFillThreads = 256;
+constexpr int kSpCountCap = QR_SPADD_COUNT_CAP;
 constexpr int kSpCap = QR_SPADD_CAP;  // staged input entries (A + B) per tile
-static_assert(kSpCap % kSpThreads == 0, "whole staging rounds per thread");
+static_assert(kSpCap % kSpFillThreads == 0 && kSpCountCap % kSpCountThreads == 0,
+              "whole staging rounds per thread");
 
 __device__ __forceinline__ uint64_t clampu(uint64_t v, uint64_t lo, uint64_t hi) {
     return v < lo ? lo : (v > hi ? hi : v);
@@ -87,17 +99,17 @@ __device__ __forceinline__ uint64_t clampu(uint64_t v, uint64_t lo, uint64_t hi)
 
 // Tile of rows [r0, r0 + rows_per_tile).  kFill = false writes counts[r];
 // kFill = true writes the merged rows at ipo[r].
-template <bool kFill>
-__global__ __launch_bounds__(kSpThreads) void k_spadd_tile(uint64_t nrows, uint32_t rows_per_tile,
+template <bool kFill, int kT, int kCap>
+__global__ __launch_bounds__(kT) void k_spadd_tile(uint64_t nrows, uint32_t rows_per_tile,
                                                            uint32_t lpr,
                                                            SpAddIn in, uint64_t *__restrict__ counts,
                                                            const uint64_t *__restrict__ ipo,
                                                            uint64_t *__restrict__ ixo,
                                                            double2 *__restrict__ dout) {
-    __shared__ uint64_t sc[kSpCap];
-    __shared__ double2 sv[kSpCap];
-    __shared__ uint64_t oc[kFill ? kSpCap : 1];
-    __shared__ double2 ov[kFill ? kSpCap : 1];
+    __shared__ uint64_t sc[kCap];
+    __shared__ double2 sv[kCap];
+    __shared__ uint64_t oc[kFill ? kCap : 1];
+    __shared__ double2 ov[kFill ? kCap : 1];
     const uint32_t tid = threadIdx.x;
     const uint64_t r0 = uint64_t(blockIdx.x) * rows_per_tile;
     if (r0 >= nrows) return;
@@ -105,15 +117,21 @@ __global__ __launch_bounds__(kSpThreads) void k_spadd_tile(uint64_t nrows, uint3
     const uint64_t a0 = clampu(in.ip1[r0], 0, in.nnz1), a1 = clampu(in.ip1[r1], a0, in.nnz1);
     const uint64_t b0 = clampu(in.ip2[r0], 0, in.nnz2), b1 = clampu(in.ip2[r1], b0, in.nnz2);
     const uint64_t ea = a1 - a0, eb = b1 - b0;
-    if (ea + eb > uint64_t(kSpCap)) {
-        // global-memory merge, one thread per row (the same per-row clamping as
-        // the count pass, so both passes agree)
-        for (uint64_t r = r0 + tid; r < r1; r += kSpThreads) {
+    if (ea + eb > uint64_t(kCap)) {
+        // global-memory merge, one thread per row.  For a CSR whose indptr
+        // ascends within nnz it gives each row what the LDS merge gives; the
+        // two passes tile the rows differently, so for a malformed one a row's
+        // count may come from the other route, and the fill stops at the
+        // row's end either way.
+        for (uint64_t r = r0 + tid; r < r1; r += kT) {
             if (kFill) {
                 uint64_t k = ipo[r];
+                const uint64_t kend = ipo[r + 1];
                 merge_row(r, in, [&](uint64_t c, double2 v) {
-                    ixo[k] = c;
-                    dout[k] = v;
+                    if (k < kend) {
+                        ixo[k] = c;
+                        dout[k] = v;
+                    }
                     ++k;
                 });
             } else {
@@ -127,12 +145,12 @@ __global__ __launch_bounds__(kSpThreads) void k_spadd_tile(uint64_t nrows, uint3
     {
         // every load of this thread in flight at once: slot k < ea is A's
         // entry a0 + k, slot ea <= k < ea + eb B's entry b0 + k - ea
-        constexpr int kPer = kSpCap / kSpThreads;
+        constexpr int kPer = kCap / kT;
         uint64_t c[kPer];
         double2 v[kPer];
 #pragma unroll
         for (int u = 0; u < kPer; ++u) {
-            const uint32_t k = tid + u * kSpThreads;
+            const uint32_t k = tid + u * kT;
             if (k < ea) {
                 c[u] = __builtin_nontemporal_load(in.ix1 + a0 + k);
                 v[u].x = __builtin_nontemporal_load(&in.d1[a0 + k].x);
@@ -145,7 +163,7 @@ __global__ __launch_bounds__(kSpThreads) void k_spadd_tile(uint64_t nrows, uint3
         }
 #pragma unroll
         for (int u = 0; u < kPer; ++u) {
-            const uint32_t k = tid + u * kSpThreads;
+            const uint32_t k = tid + u * kT;
             if (k < ea + eb) {
                 sc[k] = c[u];
                 sv[k] = cmul2(k < ea ? in.a : in.b, v[u]);
@@ -156,7 +174,7 @@ __global__ __launch_bounds__(kSpThreads) void k_spadd_tile(uint64_t nrows, uint3
     const uint64_t o0 = kFill ? ipo[r0] : 0;
     // lpr lanes per row (a power of two <= 64, so a row's lanes share a wave)
     const uint32_t lane = tid & (lpr - 1);
-    for (uint64_t r = r0 + tid / lpr; r < r1; r += kSpThreads / lpr) {
+    for (uint64_t r = r0 + tid / lpr; r < r1; r += kT / lpr) {
         // row bounds inside the staged ranges: B's entries sit after A's
         const uint32_t ia = uint32_t(clampu(in.ip1[r], a0, a1) - a0);
         const uint32_t ia1 = uint32_t(clampu(in.ip1[r + 1], a0 + ia, a1) - a0);
@@ -232,7 +250,7 @@ __global__ __launch_bounds__(kSpThreads) void k_spadd_tile(uint64_t nrows, uint3
         } else {
             uint64_t k = ipo[r] - o0 + (incl - n);
             walk([&](uint64_t col, double2 v) {
-                if (k < uint64_t(kSpCap)) {
+                if (k < uint64_t(kCap)) {
                     oc[k] = col;
                     ov[k] = v;
                 }
@@ -243,8 +261,8 @@ __global__ __launch_bounds__(kSpThreads) void k_spadd_tile(uint64_t nrows, uint3
     if (kFill) {
         __syncthreads();
         uint64_t total = ipo[r1] - o0;
-        total = total < uint64_t(kSpCap) ? total : uint64_t(kSpCap);
-        for (uint32_t k = tid; k < total; k += kSpThreads) {
+        total = total < uint64_t(kCap) ? total : uint64_t(kCap);
+        for (uint32_t k = tid; k < total; k += kT) {
             __builtin_nontemporal_store(oc[k], ixo + o0 + k);
             __builtin_nontemporal_store(ov[k].x, &dout[o0 + k].x);
             __builtin_nontemporal_store(ov[k].y, &dout[o0 + k].y);
@@ -258,12 +276,12 @@ __global__ __launch_bounds__(kSpThreads) void k_spadd_tile(uint64_t nrows, uint3
 struct SpTiling {
     uint32_t rows, lpr;
 };
-static SpTiling spadd_tiling(uint64_t nrows, uint64_t nnz1, uint64_t nnz2) {
+static SpTiling spadd_tiling(uint64_t nrows, uint64_t nnz1, uint64_t nnz2, int threads, int cap) {
     const double mean = double(nnz1 + nnz2) / double(nrows ? nrows : 1);
-    const double r = mean > 0 ? 0.85 * kSpCap / mean : double(kSpThreads);
-    const uint32_t rows = r >= kSpThreads ? uint32_t(kSpThreads) : (r < 1 ? 1u : uint32_t(r));
+    const double r = mean > 0 ? 0.85 * cap / mean : double(threads);
+    const uint32_t rows = r >= threads ? uint32_t(threads) : (r < 1 ? 1u : uint32_t(r));
     uint32_t lpr = 1;
-    while (lpr < 64 && lpr * rows < uint32_t(kSpThreads)) lpr <<= 1;
+    while (lpr < 64 && lpr * rows < uint32_t(threads)) lpr <<= 1;
     return SpTiling{rows, lpr};
 }
 
@@ -288,14 +306,14 @@ int launch_spadd_count(uint64_t nrows, const double *a, uint64_t nnz1, const uin
     const size_t cbytes = (size_t(nrows) * sizeof(uint64_t) + 255) & ~size_t(255);
     QR_HIP_CHECK(hipMallocAsync(&tmp, cbytes + tmp_bytes, s));
     uint64_t *counts = static_cast<uint64_t *>(tmp);
-    const SpTiling tl = spadd_tiling(nrows, nnz1, nnz2);
+    const SpTiling tl = spadd_tiling(nrows, nnz1, nnz2, kSpCountThreads, kSpCountCap);
     const uint64_t ntiles = (nrows + tl.rows - 1) / tl.rows;
     if (ntiles > 0x7FFFFFFFull) {
         (void)hipFreeAsync(tmp, s);
         return fail(QR_ERR_SIZE, "sparse add: too many row tiles");
     }
     hipLaunchKernelGGL(k_set_zero_u64, dim3(1), dim3(1), 0, s, ipo);
-    hipLaunchKernelGGL(k_spadd_tile<false>, dim3(unsigned(ntiles)), dim3(kSpThreads), 0, s, nrows,
+    hipLaunchKernelGGL((k_spadd_tile<false, kSpCountThreads, kSpCountCap>), dim3(unsigned(ntiles)), dim3(kSpCountThreads), 0, s, nrows,
                        tl.rows, tl.lpr, in, counts, nullptr, nullptr, nullptr);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess)
@@ -313,10 +331,10 @@ int launch_spadd_fill(uint64_t nrows, const double *a, uint64_t nnz1, const uint
                       const uint64_t *ip2, const uint64_t *ix2, const double *d2,
                       const uint64_t *ipo, uint64_t *ixo, double *dout, hipStream_t s) {
     const SpAddIn in = spadd_in(a, nnz1, ip1, ix1, d1, b, nnz2, ip2, ix2, d2);
-    const SpTiling tl = spadd_tiling(nrows, nnz1, nnz2);
+    const SpTiling tl = spadd_tiling(nrows, nnz1, nnz2, kSpFillThreads, kSpCap);
     const uint64_t ntiles = (nrows + tl.rows - 1) / tl.rows;
     if (ntiles > 0x7FFFFFFFull) return fail(QR_ERR_SIZE, "sparse add: too many row tiles");
-    hipLaunchKernelGGL(k_spadd_tile<true>, dim3(unsigned(ntiles)), dim3(kSpThreads), 0, s, nrows,
+    hipLaunchKernelGGL((k_spadd_tile<true, kSpFillThreads, kSpCap>), dim3(unsigned(ntiles)), dim3(kSpFillThreads), 0, s, nrows,
                        tl.rows, tl.lpr, in, nullptr, ipo, ixo, reinterpret_cast<double2 *>(dout));
     QR_HIP_CHECK(hipGetLastError());
     return QR_OK;

@@ -210,6 +210,35 @@ def test_spadd_bit_exact_random():
             assert np.array_equal(_bits(got[2]), _bits(ref[2]))
 
 
+def test_spadd_malformed_stays_in_bounds():
+    """A CSR handed in through new_unchecked with an indptr that descends in
+    places and runs past nnz: the count and fill passes tile the rows
+    differently and may clamp such rows differently (the entries of such rows
+    are then unspecified), but every read and write stays in bounds: the call
+    returns, with an indptr that ascends from 0 to the output's entry count."""
+    import qrusty_amd as qa
+    rng = np.random.default_rng(77)
+    rows, cols = 4000, 3000
+    lens = rng.integers(0, 60, size=rows)
+    lens[rng.choice(rows, size=40, replace=False)] = rng.integers(300, 1200, size=40)
+    indptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint64)
+    nnz = int(indptr[-1])
+    indices = np.concatenate([np.sort(rng.choice(cols, size=int(n), replace=False)) for n in lens]
+                             + [np.zeros(0, dtype=np.int64)]).astype(np.uint64)
+    data = rng.standard_normal(nnz) + 1j * rng.standard_normal(nnz)
+    bad = indptr.copy()
+    swap = rng.choice(np.arange(1, rows), size=200, replace=False)
+    bad[swap] = bad[swap] + np.uint64(5000)  # descends at the next row
+    bad[rows - 30:] += np.uint64(10**7)      # the last rows start beyond nnz
+    m_bad = qa.SpMat.new_unchecked((rows, cols), data, indices, bad)
+    m_ok = qa.SpMat.new_unchecked((rows, cols), data, indices, indptr)
+    print("[gpu] spadd malformed indptr", flush=True)
+    for a, b in [(1.0, 1.0), (0.5, -2.0)]:
+        ip, ix, dv = _host(qa.a_spmat_p_b_spmat(a, m_bad, b, m_ok))
+        assert ip[0] == 0 and np.all(np.diff(ip.astype(np.int64)) >= 0)
+        assert int(ip[-1]) == ix.size == dv.size
+
+
 def test_spadd_pauli_shift(hamiltonians):
     """H - e*I on a Pauli CSR (the Davidson shift): bit-exact vs the oracle and
     equal to the dense a*H + b*I; SpMat + and - and scale."""
