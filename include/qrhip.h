@@ -307,6 +307,9 @@ int qr_mm_read_close(qr_mm_reader *reader);
  * loads each partner tile into LDS once for all the groups that read it.
  * The library picks one from the operator's masks and the state size; the
  * environment variable QR_APPLY_FORM=1 (gather) or 2 (staged) forces it.
+ * QR_PREC_MIXED takes tiles of 2^11 rows, or of 2^10 when the launch would
+ * otherwise have fewer than 4 tiles per CU (results again equal within
+ * rounding); QR_APPLY_SMALL=0 or 1 forces the larger or the smaller.
  * ---------------------------------------------------------------------- */
 enum qr_precision { QR_PREC_FP64 = 0, QR_PREC_MIXED = 1 };
 
