@@ -789,8 +789,8 @@ def main():
                     default=None, help="one section; io (files) runs only when named")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
-              "(see DESIGN.md for how the path would shard).", file=sys.stderr)
+        print("bench.py: runs on one GPU only; launches over several GPUs are not "
+              "implemented (the path's sharding exists as a design, not as code).", file=sys.stderr)
         sys.exit(2)
 
     import torch

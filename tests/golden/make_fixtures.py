@@ -1,7 +1,7 @@
 """Extract the reference's own test Hamiltonians as data and derive golden outputs.
 
-Run in a container where the qrusty source tree is mounted (default
-/root/reference); the GPU box never runs this.  The reference files are read
+Run where the qrusty source tree is mounted (default /root/reference); the
+tests only read the fixtures it wrote, never this script.  The reference files are read
 as TEXT and parsed with regular expressions -- nothing from them is imported
 or executed:
 
