@@ -234,6 +234,31 @@ def test_apply_forms_vs_oracle(monkeypatch, kind, n, T, seed):
         monkeypatch.delenv("QR_APPLY_SMALL")
 
 
+def test_mixed_tile_size_choice(monkeypatch, hamiltonians):
+    """Complex64 applies take 2^10-row tiles when 2^11-row ones would give
+    fewer than 4 tiles per CU: one 16-qubit state (32 tiles) takes the small
+    ones, so its results equal QR_APPLY_SMALL=1 bit for bit; 300 states of it
+    (9600 tiles, more than 4 per CU on any MI355X) take the default ones and
+    equal QR_APPLY_SMALL=0.  The two sizes agree within rounding."""
+    import torch as _t
+    labels, coeffs = hamiltonians["py_H8"]  # 16 qubits
+    op = _op(labels, coeffs)
+    cus = _t.cuda.get_device_properties(0).multi_processor_count
+    for nvec, forced in [(1, "1"), (300, "0")]:
+        assert ((1 << 16) >> 11) * nvec < 4 * cus if forced == "1" else ((1 << 16) >> 11) * nvec >= 4 * cus
+        x = _states(1 << 16, nvec, seed=61)
+        print(f"[gpu] mixed tile choice nvec={nvec}", flush=True)
+        monkeypatch.delenv("QR_APPLY_SMALL", raising=False)
+        y, e = op.apply_and_expectation(x, "mixed")
+        monkeypatch.setenv("QR_APPLY_SMALL", forced)
+        yf, ef = op.apply_and_expectation(x, "mixed")
+        monkeypatch.setenv("QR_APPLY_SMALL", "1" if forced == "0" else "0")
+        yo, _ = op.apply_and_expectation(x, "mixed")
+        assert np.array_equal(y, yf) and np.array_equal(e, ef)
+        assert np.linalg.norm(yo - y) <= 1e-6 * np.linalg.norm(y)
+    monkeypatch.delenv("QR_APPLY_SMALL")
+
+
 @pytest.mark.parametrize("form", ["1", "2"])
 def test_wide_mask_instances(monkeypatch, form):
     """The 64-bit-mask kernel instances (used from 33 qubits on) on a 14-qubit
