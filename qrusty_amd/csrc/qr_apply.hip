@@ -907,14 +907,22 @@ static int launch_staged(const qr_op *op, DeviceTables *t, int slot, uint32_t nv
 //   FP64 (2^10-row tiles): faster in every case measured -- k-local
 //     1.29..1.86x, 28-qubit k-local 1.40x, Heisenberg ring 1.07..1.10x,
 //     20-qubit random (F/D 1.0..2.3) 1.11..1.36x.
-// Hence: FP64 staged whenever a group reaches outside the tile; mixed staged
-// when F >= 1.5 D for states of at least 32 MiB per vector, F >= 2 D below
-// that.  The environment variable QR_APPLY_FORM (1 gather, 2 staged)
-// overrides the choice, for tests and measurement.
-static bool prefer_staged(const qr_op *op, int L, size_t elem_bytes) {
+//   small complex64 tiles (2^10 rows; only launches of at most four tiles
+//   per CU take them: at most 16 MiB of state on 256 CUs): the gather form, faster in
+//   every case measured -- 20 qubits, one state, gather / staged: the
+//   reference's H10 1.58 / 1.64 ms, random 2000 terms 1.19 / 1.45, random
+//   300 terms 0.20 / 0.28, k-local 3000 terms 0.43 / 0.46, k-local 500
+//   terms 0.13 / 0.15, Heisenberg ring 0.026 / 0.029.
+// Hence: FP64 staged whenever a group reaches outside the tile; mixed on
+// small tiles gathered; mixed on default tiles staged when F >= 1.5 D for
+// states of at least 32 MiB per vector, F >= 2 D below that.  The
+// environment variable QR_APPLY_FORM (1 gather, 2 staged) overrides the
+// choice, for tests and measurement.
+static bool prefer_staged(const qr_op *op, int L, size_t elem_bytes, bool small_tiles) {
     const char *e = getenv("QR_APPLY_FORM");
     if (e && atoi(e) == 1) return false;
     if (e && atoi(e) == 2) return true;
+    if (small_tiles) return false;
     // gx ascends, so groups with equal bits above the tile are adjacent
     uint64_t far = 0, distinct = 0, prev = 0;
     for (uint64_t g = 0; g < op->G; ++g) {
@@ -1014,7 +1022,7 @@ static ApplyPlan make_plan(const qr_op *op, int kind) {
     using Shape = ApplyShape<kMixed, kL>;
     ApplyPlan p;
     p.L = int(op->nq) < kL ? int(op->nq) : kL;
-    p.staged = p.L == kL && prefer_staged(op, p.L, kMixed ? 8 : 16);
+    p.staged = p.L == kL && prefer_staged(op, p.L, kMixed ? 8 : 16, kind == 2);
     p.pcap = p.staged ? Shape::kPatChunkS : Shape::kPatChunk;
     p.ucap = p.staged ? Shape::kTableRowsS : Shape::kTableRows;
     // the gather form on full complex64 tiles runs the paired-row layout
